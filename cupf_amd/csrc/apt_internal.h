@@ -1,0 +1,89 @@
+// apt_internal.h -- device-side argument blocks shared by the kernels and the
+// C-ABI layer of cupf_amd.  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cupf {
+
+// LS = -0.995 (3DgrainMPI/APTPhaseField.cu:25) is a double literal in the
+// reference: every `phi > LS` test compares in double.
+constexpr double kLS = -0.995;
+// Slot kill threshold (3DgrainMPI/APTPhaseField.cu:341), also a double compare.
+constexpr double kKill = -0.9999;
+
+// Everything one APT step launch reads.  Passed by value (kernel argument
+// segment -> SGPRs), so no constant-memory upload per step.
+struct StepArgs {
+    const float* ph;       // source buffer, K slot planes
+    const int* arg;
+    float* ph_new;         // destination buffer
+    int* arg_new;
+    const float* x;
+    const float* y;
+    const float* z;
+    const float* R12;      // orientation rows, 12 floats per grain (9 used)
+    const float* psi_mac;  // type-4 thermal field
+    unsigned long long* overflow;  // count of points with > K distinct ids
+    unsigned* fix_list;    // interior points left to the fix-up pass (capacity: interior points)
+    unsigned* fix_count;   // [0] entries in fix_list, [1] fix-up workgroups finished
+    int fnx, fny, fnz;
+    int L;                 // slot stride fnx*fny*fnz
+    int ncolx, ncols;      // interior columns: (fnx-2), (fnx-2)*(fny-2)
+    int nzchunk, kchunk;   // z-chunks per column, planes per chunk
+    int nblk_col;          // column blocks
+    int bcx, bcy, bcz;
+    int num_theta;
+    int thermal_type;
+    int use_iuc;
+    float t;
+    float dt, hi, lamd, L_cp, kin_delta, eps;
+    float G, R, uc, uc0, z0, r0, lyd, lzd, V, mp_len, x0, order;
+    float tan_angle, cos_angle;
+    int Nx, Ny, Nz;
+    float X0_mac, Y0_mac, Z0_mac, Dx_mac;
+};
+
+// Nucleation launch block (3DgrainMPI/APTPhaseField.cu:459-536).
+struct NuclArgs {
+    float* ph;
+    int* arg;
+    int* status;
+    const float* x;
+    const float* y;
+    const float* z;
+    const float* T3d;
+    const float* ball;     // (2p+1)^3 tanh template
+    unsigned long long* events;
+    int fnx, fny, fnz, L, K;
+    int cnx, cny, cnz, p;
+    int use_iuc;
+    float t, dt;
+    float uc0, undcool_mean, undcool_std, nuc_Nmax, nuc_Nmax0, nuc_rad;
+    int Nx, Ny, Nz, Nt;
+    float X0_mac, Y0_mac, Z0_mac, t0_mac, Dx_mac, Dt_mac;
+    uint64_t seed;
+    uint64_t call;
+};
+
+// Launchers (apt_kernels.hip).  All enqueue on `stream` and return the
+// hipError_t of the launch.
+hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream);
+hipError_t launch_bc_fill(float* ph, int* arg, int fnx, int fny, int fnz, int K, int bcx,
+                          int bcy, int bcz, hipStream_t stream);
+hipError_t launch_collect(const float* ph, const int* arg, float* phi, int* alpha, int L, int K,
+                          int clear_alpha, hipStream_t stream);
+hipError_t launch_ini(float* ph0, int* arg0, float* ph1, int* arg1, const float* phi0,
+                      const int* alpha0, int L, int K, hipStream_t stream);
+hipError_t launch_plane_sums(const float* ph, long long* sums, int fnx, int fny, int fnz, int K,
+                             hipStream_t stream);
+hipError_t launch_move_frame(float* ph_old, int* arg_old, float* ph_new, int* arg_new, float* z,
+                             float* zbuf, const int* alpha, int* alpha_full, int* loss_area,
+                             int move_count, int fnx, int fny, int fnz, int K, int num_theta,
+                             hipStream_t stream);
+hipError_t launch_nucl_status(const float* phi0, int* status, int cnx, int cny, int cnz, int p,
+                              int fnx, int fny, hipStream_t stream);
+hipError_t launch_nucl(const NuclArgs& a, hipStream_t stream);
+
+}  // namespace cupf

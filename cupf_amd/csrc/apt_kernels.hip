@@ -1,0 +1,986 @@
+// apt_kernels.hip -- gfx950 kernels of the APT multi-grain phase-field step.
+//
+// Field layout (3DgrainMPI/devicefunc.cu_inl:24-29): SoA slot planes,
+// element (i,j,k,slot) at i + j*fnx + k*fnx*fny + slot*L, phi float32 and
+// grain id int32 in separate arrays, -1 = empty slot.
+//
+// Arithmetic follows the reference source operation by operation (build with
+// -ffp-contract=off): float everywhere except the reference's implicit double
+// promotions (the 1e6 thermal factors, LS=-0.995 and the -0.9999 kill test).
+#include "apt_internal.h"
+
+namespace cupf {
+
+namespace {
+
+constexpr int kBlock = 256;        // 4 wave64 per workgroup
+constexpr int kFixupBlocks = 512;  // fix-up grid: 2 workgroups per CU
+
+// Halo index -> interior source index along one axis (APTsetBC3D,
+// 3DgrainMPI/APTPhaseField.cu:111-112,134-135,157-158): no-flux mirrors
+// out[0]=in[2], out[n-1]=in[n-3]; periodic wraps out[0]=in[n-2], out[n-1]=in[1].
+__device__ __forceinline__ int bc_map(int q, int n, int periodic)
+{
+    if (q == 0) return periodic ? n - 2 : 2;
+    if (q == n - 1) return periodic ? 1 : n - 3;
+    return q;
+}
+
+// kine_ani (3DgrainMPI/APTPhaseField.cu:36-52) with a row-major rotation;
+// each row is (m0*ux + m1*uy) + m2*uz, which equals the reference's angle
+// form bit for bit (a - b*c == a + (-b)*c; the "+ 0" only flips signs of zeros
+// that are squared away).
+__device__ __forceinline__ float kine_ani(float ux, float uy, float uz, float m0, float m1,
+                                          float m2, float m3, float m4, float m5, float m6,
+                                          float m7, float m8, float a_s, float epsv, float eps)
+{
+    float ux2 = m0 * ux + m1 * uy + m2 * uz;
+    ux2 = ux2 * ux2;
+    float uy2 = m3 * ux + m4 * uy + m5 * uz;
+    uy2 = uy2 * uy2;
+    float uz2 = m6 * ux + m7 * uy + m8 * uz;
+    uz2 = uz2 * uz2;
+    float mag = ux2 + uy2 + uz2;
+    float mag2 = mag * mag;
+    if (mag > eps) return a_s * (1.0f + epsv * (ux2 * ux2 + uy2 * uy2 + uz2 * uz2) / mag2);
+    return 1.0f;
+}
+
+// interp3Dtemperature (3DgrainMPI/devicefunc.cu_inl:85-125), indices clamped
+// into the macro grid (the reference reads out of bounds there).
+__device__ __forceinline__ float interp3(const float* __restrict__ u, float x, float y, float z,
+                                         int Nx, int Ny, int Nz, float Dx)
+{
+    int kx = (int)(x / Dx), ky = (int)(y / Dx), kz = (int)(z / Dx);
+    float dx = x / Dx - kx, dy = y / Dx - ky, dz = z / Dx - kz;
+    if (kx == Nx - 1) { kx = Nx - 2; dx = 1.0f; }
+    if (ky == Ny - 1) { ky = Ny - 2; dy = 1.0f; }
+    if (kz == Nz - 1) { kz = Nz - 2; dz = 1.0f; }
+    kx = min(max(kx, 0), Nx - 2);
+    ky = min(max(ky, 0), Ny - 2);
+    kz = min(max(kz, 0), Nz - 2);
+    int o = kx + ky * Nx + kz * Nx * Ny;
+    int on = kx + ky * Nx + (kz + 1) * Nx * Ny;
+    return ((1.0f - dx) * (1.0f - dy) * u[o] + (1.0f - dx) * dy * u[o + Nx]
+            + dx * (1.0f - dy) * u[o + 1] + dx * dy * u[o + Nx + 1]) * (1.0f - dz)
+         + ((1.0f - dx) * (1.0f - dy) * u[on] + (1.0f - dx) * dy * u[on + Nx]
+            + dx * (1.0f - dy) * u[on + 1] + dx * dy * u[on + Nx + 1]) * dz;
+}
+
+// interp4Dtemperature (3DgrainMPI/devicefunc.cu_inl:32-83), clamped likewise.
+__device__ __forceinline__ float interp4(const float* __restrict__ u, float x, float y, float z,
+                                         float t, int Nx, int Ny, int Nz, int Nt, float Dx,
+                                         float Dt)
+{
+    int kx = (int)(x / Dx), ky = (int)(y / Dx), kz = (int)(z / Dx), kt = (int)(t / Dt);
+    float dx = x / Dx - kx, dy = y / Dx - ky, dz = z / Dx - kz, dt = t / Dt - kt;
+    if (kx == Nx - 1) { kx = Nx - 2; dx = 1.0f; }
+    if (ky == Ny - 1) { ky = Ny - 2; dy = 1.0f; }
+    if (kz == Nz - 1) { kz = Nz - 2; dz = 1.0f; }
+    if (kt == Nt - 1) { kt = Nt - 2; dt = 1.0f; }
+    kx = min(max(kx, 0), Nx - 2);
+    ky = min(max(ky, 0), Ny - 2);
+    kz = min(max(kz, 0), Nz - 2);
+    kt = min(max(kt, 0), Nt - 2);
+    const int NN = Nx * Ny * Nz;
+    int o = kx + ky * Nx + kz * Nx * Ny + kt * NN;
+    int on = kx + ky * Nx + (kz + 1) * Nx * Ny + kt * NN;
+    float up = ((1.0f - dx) * (1.0f - dy) * u[o] + (1.0f - dx) * dy * u[o + Nx]
+                + dx * (1.0f - dy) * u[o + 1] + dx * dy * u[o + Nx + 1]) * (1.0f - dz)
+             + ((1.0f - dx) * (1.0f - dy) * u[on] + (1.0f - dx) * dy * u[on + Nx]
+                + dx * (1.0f - dy) * u[on + 1] + dx * dy * u[on + Nx + 1]) * dz;
+    o += NN;
+    on += NN;
+    float un = ((1.0f - dx) * (1.0f - dy) * u[o] + (1.0f - dx) * dy * u[o + Nx]
+                + dx * (1.0f - dy) * u[o + 1] + dx * dy * u[o + Nx + 1]) * (1.0f - dz)
+             + ((1.0f - dx) * (1.0f - dy) * u[on] + (1.0f - dx) * dy * u[on + Nx]
+                + dx * (1.0f - dy) * u[on + 1] + dx * dy * u[on + Nx + 1]) * dz;
+    return (1.0f - dt) * up + dt * un;
+}
+
+// Thermal value of one point (3DgrainMPI/APTPhaseField.cu:278-327;
+// type 5: 3Dgrain/APTPhaseField.cu:303).  Returns true where the reference
+// `continue`s past the slot, leaving the destination untouched.
+__device__ __forceinline__ bool thermal_T(const StepArgs& a, int i, int j, int k, float& T)
+{
+    const float t = a.t;
+    const float zk = a.z[k];
+    if (a.use_iuc) {
+        T = -a.uc0 * 1e6 * t;
+        return false;
+    }
+    switch (a.thermal_type) {
+    case 1:
+        T = a.G * (zk - a.z0) - a.uc * 1e6 * t;
+        return false;
+    case 2: {
+        const float yj = a.y[j];
+        float dist = sqrtf((yj - 0.5f * a.lyd) * (yj - 0.5f * a.lyd)
+                           + (zk - a.z0 - a.lzd) * (zk - a.z0 - a.lzd)) - a.r0;
+        T = -a.G * dist - a.uc * 1e6 * t;
+        return false;
+    }
+    case 3: {
+        const float xi = a.x[i], yj = a.y[j];
+        float lm = (a.r0 - a.z0) / a.tan_angle;
+        float x_start = 0;
+        float xl = xi - x_start;
+        float r0_x = a.z0 + (a.r0 - a.z0) * xl / lm;
+        float z_tilt = a.z0 + a.lzd - zk;
+        float dist = a.cos_angle * (sqrtf((yj - 0.5f * a.lyd) * (yj - 0.5f * a.lyd)
+                                          + z_tilt * z_tilt) - r0_x);
+        if (xl > lm + a.V * t * 1e6) return true;
+        T = -a.G * dist - a.uc * 1e6 * t;
+        return false;
+    }
+    case 4: {
+        const float xi = a.x[i], yj = a.y[j];
+        float xq = xi - a.V * t * 1e6;
+        if (xq > a.mp_len + a.x0 || xq < 0.0f) return true;
+        float v = interp3(a.psi_mac, xq - a.X0_mac, yj - a.Y0_mac, zk - a.Z0_mac, a.Nx, a.Ny,
+                          a.Nz, a.Dx_mac);
+        T = (a.order == 100) ? -v : -a.G * v;
+        return false;
+    }
+    case 5:
+        T = a.G * (zk - a.R * 1e6 * t - 2);
+        return false;
+    default:
+        T = 0.0f;
+        return false;
+    }
+}
+
+// Gather state of one point.  la: slot ids in first-appearance order (no
+// holes, n filled).  pm[s]: 4-bit nibble q holds the neighbour slot p whose id
+// landed in slot q at stencil row s (D,B,L,C,R,T,U), 15 = none; the last
+// matching p wins, as the reference's overwrite does.
+template <int K>
+struct Gather {
+    int la[K];
+    uint32_t pm[7];
+    int n;
+    bool ovf;
+};
+
+template <int K>
+__device__ __forceinline__ void gather_init(Gather<K>& g)
+{
+#pragma unroll
+    for (int q = 0; q < K; ++q) g.la[q] = -1;
+#pragma unroll
+    for (int s = 0; s < 7; ++s) g.pm[s] = 0xffffffffu;
+    g.n = 0;
+    g.ovf = false;
+}
+
+// Pass A for one neighbour point (3DgrainMPI/APTPhaseField.cu:215-232): load
+// its K ids and probe-insert them.  la has no holes, so the probe returns the
+// matching slot or slot n.  S is the stencil row (-1: 27-point extra point).
+template <int K, int S>
+__device__ __forceinline__ void scan_point(const int* __restrict__ arg, int base, int L,
+                                           Gather<K>& g)
+{
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        const int id = arg[base + p * L];
+        if (id == -1) continue;
+        int pos = g.n;
+#pragma unroll
+        for (int q = 0; q < K; ++q) pos = (g.la[q] == id) ? q : pos;
+        if (pos >= K) {
+            g.ovf = true;
+            continue;
+        }
+#pragma unroll
+        for (int q = 0; q < K; ++q) g.la[q] = (q == pos) ? id : g.la[q];
+        g.n += (pos == g.n) ? 1 : 0;
+        if (S >= 0) {
+            const uint32_t sh = 4u * (uint32_t)pos;
+            g.pm[S] = (g.pm[S] & ~(15u << sh)) | ((uint32_t)p << sh);
+        }
+    }
+    // Keep one neighbour point's loads in flight per lane; occupancy supplies
+    // the memory parallelism instead of registers.
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// Neighbour bases of an interior point through the per-axis halo maps.
+struct Nbr {
+    int xs[3], ys[3], zs[3];  // (minus, centre, plus) per axis, already mapped
+};
+
+__device__ __forceinline__ Nbr make_nbr(const StepArgs& a, int i, int j, int k)
+{
+    Nbr n;
+    n.xs[0] = bc_map(i - 1, a.fnx, a.bcx); n.xs[1] = i; n.xs[2] = bc_map(i + 1, a.fnx, a.bcx);
+    n.ys[0] = bc_map(j - 1, a.fny, a.bcy); n.ys[1] = j; n.ys[2] = bc_map(j + 1, a.fny, a.bcy);
+    n.zs[0] = bc_map(k - 1, a.fnz, a.bcz); n.zs[1] = k; n.zs[2] = bc_map(k + 1, a.fnz, a.bcz);
+    return n;
+}
+
+__device__ __forceinline__ int nbase(const StepArgs& a, const Nbr& nb, int xi, int yi, int zi)
+{
+    return nb.xs[xi] + nb.ys[yi] * a.fnx + nb.zs[zi] * a.fnx * a.fny;
+}
+
+// Stencil row bases D,B,L,C,R,T,U (stencil = -3..3, 3DgrainMPI/APTPhaseField.cu:211-214).
+__device__ __forceinline__ void stencil_bases(const StepArgs& a, const Nbr& nb, int (&b)[7])
+{
+    b[0] = nbase(a, nb, 1, 1, 0);
+    b[1] = nbase(a, nb, 1, 0, 1);
+    b[2] = nbase(a, nb, 0, 1, 1);
+    b[3] = nbase(a, nb, 1, 1, 1);
+    b[4] = nbase(a, nb, 2, 1, 1);
+    b[5] = nbase(a, nb, 1, 2, 1);
+    b[6] = nbase(a, nb, 1, 1, 2);
+}
+
+// Pass A over the whole neighbourhood in the reference's scan order.
+template <int K, bool G27>
+__device__ __forceinline__ void scan_all(const StepArgs& a, const Nbr& nb, Gather<K>& g)
+{
+    if (!G27) {
+        scan_point<K, 0>(a.arg, nbase(a, nb, 1, 1, 0), a.L, g);
+        scan_point<K, 1>(a.arg, nbase(a, nb, 1, 0, 1), a.L, g);
+        scan_point<K, 2>(a.arg, nbase(a, nb, 0, 1, 1), a.L, g);
+        scan_point<K, 3>(a.arg, nbase(a, nb, 1, 1, 1), a.L, g);
+        scan_point<K, 4>(a.arg, nbase(a, nb, 2, 1, 1), a.L, g);
+        scan_point<K, 5>(a.arg, nbase(a, nb, 1, 2, 1), a.L, g);
+        scan_point<K, 6>(a.arg, nbase(a, nb, 1, 1, 2), a.L, g);
+    } else {
+        // z-major 27-point order (3Dgrain/APTPhaseField.cu:241-268); the stencil
+        // rows keep phi: D=(1,1,0) B=(1,0,1) L=(0,1,1) C R T U=(1,1,2).
+#define CUPF_P(XI, YI, ZI, S) scan_point<K, S>(a.arg, nbase(a, nb, XI, YI, ZI), a.L, g)
+        CUPF_P(0, 0, 0, -1); CUPF_P(1, 0, 0, -1); CUPF_P(2, 0, 0, -1);
+        CUPF_P(0, 1, 0, -1); CUPF_P(1, 1, 0, 0);  CUPF_P(2, 1, 0, -1);
+        CUPF_P(0, 2, 0, -1); CUPF_P(1, 2, 0, -1); CUPF_P(2, 2, 0, -1);
+        CUPF_P(0, 0, 1, -1); CUPF_P(1, 0, 1, 1);  CUPF_P(2, 0, 1, -1);
+        CUPF_P(0, 1, 1, 2);  CUPF_P(1, 1, 1, 3);  CUPF_P(2, 1, 1, 4);
+        CUPF_P(0, 2, 1, -1); CUPF_P(1, 2, 1, 5);  CUPF_P(2, 2, 1, -1);
+        CUPF_P(0, 0, 2, -1); CUPF_P(1, 0, 2, -1); CUPF_P(2, 0, 2, -1);
+        CUPF_P(0, 1, 2, -1); CUPF_P(1, 1, 2, 6);  CUPF_P(2, 1, 2, -1);
+        CUPF_P(0, 2, 2, -1); CUPF_P(1, 2, 2, -1); CUPF_P(2, 2, 2, -1);
+#undef CUPF_P
+    }
+}
+
+// Right-hand side and explicit Euler update of one filled slot
+// (3DgrainMPI/APTPhaseField.cu:250-340).  repul is the ascending sum of
+// 0.25*(phiC_p+1)^2 over the other filled slots (empty ones add exactly +0).
+__device__ __forceinline__ float slot_update(const StepArgs& a, int gid, float phD, float phB,
+                                             float phL, float phC, float phR, float phT,
+                                             float phU, float repul, float Up, float a_s,
+                                             float epsv)
+{
+    const float phxn = (phR - phL) * 0.5f;
+    const float phyn = (phT - phB) * 0.5f;
+    const float phzn = (phU - phD) * 0.5f;
+    float m0 = 1.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f, m4 = 1.0f, m5 = 0.0f, m6 = 0.0f, m7 = 0.0f,
+          m8 = 1.0f;
+    if ((double)phC > kLS) {
+        const int tid = gid <= a.num_theta ? gid : (gid % a.num_theta) + 1;
+        const float4* r = reinterpret_cast<const float4*>(a.R12 + 12 * tid);
+        const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+        m0 = r0.x; m1 = r0.y; m2 = r0.z; m3 = r0.w;
+        m4 = r1.x; m5 = r1.y; m6 = r1.z; m7 = r1.w;
+        m8 = r2.x;
+    }
+    const float A2 = kine_ani(phxn, phyn, phzn, m0, m1, m2, m3, m4, m5, m6, m7, m8, a_s, epsv, a.eps);
+    const float diff = phR + phL + phT + phB + phU + phD - 6.0f * phC;
+    const float omc = 1.0f - phC * phC;
+    const float rhs = diff * a.hi * a.hi + omc * phC
+                    - a.lamd * Up * (omc * omc - 6.0f * (phC + 1.0f) * repul);
+    const float dphi = rhs / A2;
+    return phC + a.dt * dphi;
+}
+
+// Pass B: right-hand side and explicit Euler update of every slot of one point
+// (3DgrainMPI/APTPhaseField.cu:238-351), loading exactly the phi values each
+// filled slot needs.
+template <int K>
+__device__ __forceinline__ void rhs_point(const StepArgs& a, int i, int j, int k,
+                                          const int (&b)[7], const Gather<K>& g, float a_s,
+                                          float epsv)
+{
+    float* __restrict__ phn = a.ph_new;
+    int* __restrict__ agn = a.arg_new;
+    const int C = b[3];
+    const int L = a.L;
+    if (g.n == 0) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            agn[C + q * L] = -1;
+            phn[C + q * L] = -1.0f;
+        }
+        return;
+    }
+    float T;
+    const bool skip = thermal_T(a, i, j, k, T);
+    const float Up = T / a.L_cp;
+    // centre phi of every slot first: the repulsion sums over the other slots
+    float pc[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const uint32_t p = (g.pm[3] >> (4 * q)) & 15u;
+        pc[q] = (q < g.n && p != 15u) ? a.ph[C + (int)p * L] : -1.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const int o = C + q * L;
+        if (q >= g.n) {
+            agn[o] = -1;
+            phn[o] = -1.0f;
+            continue;
+        }
+        if (skip) continue;
+        float v[7];
+#pragma unroll
+        for (int s = 0; s < 7; ++s) {
+            if (s == 3) {
+                v[s] = pc[q];
+            } else {
+                const uint32_t p = (g.pm[s] >> (4 * q)) & 15u;
+                v[s] = (p != 15u) ? a.ph[b[s] + (int)p * L] : -1.0f;
+            }
+        }
+        const int gid = g.la[q];
+        float repul = 0.0f;
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+            if (p != q && p < g.n) {
+                const float w = pc[p] + 1.0f;
+                repul += 0.25f * w * w;
+            }
+        }
+        const float nv = slot_update(a, gid, v[0], v[1], v[2], v[3], v[4], v[5], v[6], repul, Up,
+                                     a_s, epsv);
+        phn[o] = nv;
+        agn[o] = ((double)nv < kKill) ? -1 : gid;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Overflow policy for a point whose neighbourhood holds more than K distinct
+// ids (undefined behaviour in the reference, 3DgrainMPI/APTPhaseField.cu:222-231).
+// Kept set: the K ids ranked first by (largest phi over their entries, then
+// earliest entry reaching it).  Computed in one streaming pass over the
+// entries with a K-candidate list: a newcomer replaces the worst candidate
+// only if its phi is strictly larger.  The worst candidate's key never
+// decreases, so no id left outside can outrank it and the list ends as the
+// exact top K.  Kept ids then take slots in first-appearance order (pass A
+// again, filtered).
+template <int K>
+struct TopK {
+    int id[K];
+    float mx[K];
+    int pos[K];
+    int n;
+};
+
+template <int K>
+__device__ __forceinline__ void topk_push(TopK<K>& c, int id, float v, int pos)
+{
+    int hit = -1;
+#pragma unroll
+    for (int q = 0; q < K; ++q) hit = (q < c.n && c.id[q] == id) ? q : hit;
+    if (hit >= 0) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            if (q == hit && v > c.mx[q]) {
+                c.mx[q] = v;
+                c.pos[q] = pos;
+            }
+        }
+        return;
+    }
+    if (c.n < K) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            if (q == c.n) {
+                c.id[q] = id;
+                c.mx[q] = v;
+                c.pos[q] = pos;
+            }
+        }
+        ++c.n;
+        return;
+    }
+    // worst candidate: smallest phi, latest position among equals
+    int w = 0;
+#pragma unroll
+    for (int q = 1; q < K; ++q) {
+        const bool worse = c.mx[q] < c.mx[w] || (c.mx[q] == c.mx[w] && c.pos[q] > c.pos[w]);
+        w = worse ? q : w;
+    }
+    float wm = c.mx[0];
+#pragma unroll
+    for (int q = 1; q < K; ++q) wm = (q == w) ? c.mx[q] : wm;
+    if (v > wm) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            if (q == w) {
+                c.id[q] = id;
+                c.mx[q] = v;
+                c.pos[q] = pos;
+            }
+        }
+    }
+}
+
+template <int K, int P>
+__device__ __forceinline__ void topk_point(const StepArgs& a, int base, TopK<K>& c)
+{
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        const int id = a.arg[base + p * a.L];
+        if (id == -1) continue;
+        topk_push(c, id, a.ph[base + p * a.L], P * K + p);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// Pass A restricted to the kept ids.
+template <int K, int S>
+__device__ __forceinline__ void scan_point_kept(const int* __restrict__ arg, int base, int L,
+                                                Gather<K>& g, const TopK<K>& kept)
+{
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        const int id = arg[base + p * L];
+        if (id == -1) continue;
+        bool ok = false;
+#pragma unroll
+        for (int u = 0; u < K; ++u) ok |= (kept.id[u] == id);
+        if (!ok) continue;
+        int pos = g.n;
+#pragma unroll
+        for (int q = 0; q < K; ++q) pos = (g.la[q] == id) ? q : pos;
+#pragma unroll
+        for (int q = 0; q < K; ++q) g.la[q] = (q == pos) ? id : g.la[q];
+        g.n += (pos == g.n) ? 1 : 0;
+        if (S >= 0) {
+            const uint32_t sh = 4u * (uint32_t)pos;
+            g.pm[S] = (g.pm[S] & ~(15u << sh)) | ((uint32_t)p << sh);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int K, bool G27>
+__device__ __forceinline__ void gather_overflow(const StepArgs& a, const Nbr& nb, Gather<K>& g)
+{
+    TopK<K> c;
+    c.n = 0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        c.id[q] = -1;
+        c.mx[q] = 0.0f;
+        c.pos[q] = 0;
+    }
+    gather_init(g);
+    if (!G27) {
+        topk_point<K, 0>(a, nbase(a, nb, 1, 1, 0), c);
+        topk_point<K, 1>(a, nbase(a, nb, 1, 0, 1), c);
+        topk_point<K, 2>(a, nbase(a, nb, 0, 1, 1), c);
+        topk_point<K, 3>(a, nbase(a, nb, 1, 1, 1), c);
+        topk_point<K, 4>(a, nbase(a, nb, 2, 1, 1), c);
+        topk_point<K, 5>(a, nbase(a, nb, 1, 2, 1), c);
+        topk_point<K, 6>(a, nbase(a, nb, 1, 1, 2), c);
+        scan_point_kept<K, 0>(a.arg, nbase(a, nb, 1, 1, 0), a.L, g, c);
+        scan_point_kept<K, 1>(a.arg, nbase(a, nb, 1, 0, 1), a.L, g, c);
+        scan_point_kept<K, 2>(a.arg, nbase(a, nb, 0, 1, 1), a.L, g, c);
+        scan_point_kept<K, 3>(a.arg, nbase(a, nb, 1, 1, 1), a.L, g, c);
+        scan_point_kept<K, 4>(a.arg, nbase(a, nb, 2, 1, 1), a.L, g, c);
+        scan_point_kept<K, 5>(a.arg, nbase(a, nb, 1, 2, 1), a.L, g, c);
+        scan_point_kept<K, 6>(a.arg, nbase(a, nb, 1, 1, 2), a.L, g, c);
+    } else {
+#define CUPF_T(XI, YI, ZI, P) topk_point<K, P>(a, nbase(a, nb, XI, YI, ZI), c)
+        CUPF_T(0, 0, 0, 0);  CUPF_T(1, 0, 0, 1);  CUPF_T(2, 0, 0, 2);
+        CUPF_T(0, 1, 0, 3);  CUPF_T(1, 1, 0, 4);  CUPF_T(2, 1, 0, 5);
+        CUPF_T(0, 2, 0, 6);  CUPF_T(1, 2, 0, 7);  CUPF_T(2, 2, 0, 8);
+        CUPF_T(0, 0, 1, 9);  CUPF_T(1, 0, 1, 10); CUPF_T(2, 0, 1, 11);
+        CUPF_T(0, 1, 1, 12); CUPF_T(1, 1, 1, 13); CUPF_T(2, 1, 1, 14);
+        CUPF_T(0, 2, 1, 15); CUPF_T(1, 2, 1, 16); CUPF_T(2, 2, 1, 17);
+        CUPF_T(0, 0, 2, 18); CUPF_T(1, 0, 2, 19); CUPF_T(2, 0, 2, 20);
+        CUPF_T(0, 1, 2, 21); CUPF_T(1, 1, 2, 22); CUPF_T(2, 1, 2, 23);
+        CUPF_T(0, 2, 2, 24); CUPF_T(1, 2, 2, 25); CUPF_T(2, 2, 2, 26);
+#undef CUPF_T
+#define CUPF_K(XI, YI, ZI, S) scan_point_kept<K, S>(a.arg, nbase(a, nb, XI, YI, ZI), a.L, g, c)
+        CUPF_K(0, 0, 0, -1); CUPF_K(1, 0, 0, -1); CUPF_K(2, 0, 0, -1);
+        CUPF_K(0, 1, 0, -1); CUPF_K(1, 1, 0, 0);  CUPF_K(2, 1, 0, -1);
+        CUPF_K(0, 2, 0, -1); CUPF_K(1, 2, 0, -1); CUPF_K(2, 2, 0, -1);
+        CUPF_K(0, 0, 1, -1); CUPF_K(1, 0, 1, 1);  CUPF_K(2, 0, 1, -1);
+        CUPF_K(0, 1, 1, 2);  CUPF_K(1, 1, 1, 3);  CUPF_K(2, 1, 1, 4);
+        CUPF_K(0, 2, 1, -1); CUPF_K(1, 2, 1, 5);  CUPF_K(2, 2, 1, -1);
+        CUPF_K(0, 0, 2, -1); CUPF_K(1, 0, 2, -1); CUPF_K(2, 0, 2, -1);
+        CUPF_K(0, 1, 2, -1); CUPF_K(1, 1, 2, 6);  CUPF_K(2, 1, 2, -1);
+        CUPF_K(0, 2, 2, -1); CUPF_K(1, 2, 2, -1); CUPF_K(2, 2, 2, -1);
+#undef CUPF_K
+    }
+}
+
+// One interior point: pass A, then pass B.  A point whose neighbourhood
+// overflows K slots is left untouched here and returned as 1; the fix-up
+// kernel redoes it under the overflow policy.
+template <int K, bool G27>
+__device__ __forceinline__ int apt_point(const StepArgs& a, int i, int j, int k, float a_s,
+                                         float epsv)
+{
+    const Nbr nb = make_nbr(a, i, j, k);
+    Gather<K> g;
+    gather_init(g);
+    scan_all<K, G27>(a, nb, g);
+    if (g.ovf) return 1;
+    int b[7];
+    stencil_bases(a, nb, b);
+    rhs_point<K>(a, i, j, k, b, g, a_s, epsv);
+    return 0;
+}
+
+// The same point under the overflow policy (fix-up kernel only).
+template <int K, bool G27>
+__device__ __forceinline__ void apt_point_overflow(const StepArgs& a, int i, int j, int k,
+                                                   float a_s, float epsv)
+{
+    const Nbr nb = make_nbr(a, i, j, k);
+    Gather<K> g;
+    gather_overflow<K, G27>(a, nb, g);
+    int b[7];
+    stencil_bases(a, nb, b);
+    rhs_point<K>(a, i, j, k, b, g, a_s, epsv);
+}
+
+// Workgroup id -> logical block, so that consecutive logical blocks land on
+// the same XCD (hardware dispatches workgroup b to XCD b % 8): the B/T
+// neighbour rows shared by adjacent column blocks then stay in one L2.
+__device__ __forceinline__ int xcd_logical_block(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
+// APTrhs_psi (3DgrainMPI/APTPhaseField.cu:178-354).  One lane owns one
+// interior (i,j) column and marches a chunk of z planes, so the planes it
+// reads as U are re-read as C and D from L1/L2 on the next iterations.  The
+// halo is never read: neighbour indices go through the per-axis boundary
+// maps, which replaces the separate APTsetBC3D pass.
+template <int K, bool G27>
+__global__ __launch_bounds__(kBlock) void apt_step(StepArgs a)
+{
+    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int zc = lb / a.nblk_col;
+    const int cb = lb - zc * a.nblk_col;
+    const int c = cb * kBlock + threadIdx.x;
+    const bool active = c < a.ncols;
+    const int j = 1 + c / a.ncolx;
+    const int i = 1 + (c - (j - 1) * a.ncolx);
+    const int k0 = 1 + zc * a.kchunk;
+    const int k1 = active ? min(k0 + a.kchunk, a.fnz - 1) : k0;
+    const float a_s = 1.0f + 3.0f * a.kin_delta;
+    const float epsv = -4.0f * a.kin_delta / a_s;
+    const int A = a.fnx * a.fny;
+    for (int k = k0; k < k1; ++k) {
+        const int ovf = apt_point<K, G27>(a, i, j, k, a_s, epsv);
+        // Append overflowed points to the fix-up list: one atomic per wave.
+        const unsigned long long m = __ballot(ovf);
+        if (m != 0ull) {
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((long long)m) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(a.fix_count, (unsigned)__popcll(m));
+            base = __shfl(base, leader);
+            if (ovf) {
+                const unsigned rank = (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+                a.fix_list[base + rank] = (unsigned)(i + j * a.fnx + k * A);
+            }
+        }
+    }
+}
+
+// Fix-up pass for the points of the last apt_step launch whose neighbourhood
+// overflowed K slots.  The last workgroup to finish adds the count to the
+// overflow diagnostic and rearms the list for the next step.
+template <int K, bool G27>
+__global__ __launch_bounds__(kBlock) void apt_fixup(StepArgs a)
+{
+    const unsigned n = __atomic_load_n(a.fix_count, __ATOMIC_RELAXED);
+    const float a_s = 1.0f + 3.0f * a.kin_delta;
+    const float epsv = -4.0f * a.kin_delta / a_s;
+    const int A = a.fnx * a.fny;
+    for (unsigned q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+        const int C = (int)a.fix_list[q];
+        const int k = C / A;
+        const int r = C - k * A;
+        const int j = r / a.fnx;
+        const int i = r - j * a.fnx;
+        apt_point_overflow<K, G27>(a, i, j, k, a_s, epsv);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(a.fix_count + 1, 1u) == gridDim.x - 1) {
+            atomicAdd(a.overflow, (unsigned long long)n);
+            atomicExch(a.fix_count, 0u);
+            atomicExch(a.fix_count + 1, 0u);
+        }
+    }
+}
+
+// Halo fill (APTsetBC3D, 3DgrainMPI/APTPhaseField.cu:93-174).  Each halo cell
+// is written exactly once, from its per-axis mapped interior point, so the
+// result is deterministic at edges and corners (the reference races there)
+// and equals a z-then-y-then-x sequential face fill.
+__global__ __launch_bounds__(kBlock) void bc_fill(float* ph, int* arg, int fnx, int fny, int fnz,
+                                                  int L, int bcx, int bcy, int bcz, int nhalo)
+{
+    int h = blockIdx.x * kBlock + threadIdx.x;
+    if (h >= nhalo) return;
+    const int s = blockIdx.y;
+    const int A = fnx * fny;
+    int i, j, k;
+    if (h < 2 * A) {
+        k = h < A ? 0 : fnz - 1;
+        const int r = h < A ? h : h - A;
+        j = r / fnx;
+        i = r - j * fnx;
+    } else {
+        h -= 2 * A;
+        const int By = fnx * (fnz - 2);
+        if (h < 2 * By) {
+            j = h < By ? 0 : fny - 1;
+            const int r = h < By ? h : h - By;
+            k = 1 + r / fnx;
+            i = r - (k - 1) * fnx;
+        } else {
+            h -= 2 * By;
+            const int Bx = (fny - 2) * (fnz - 2);
+            i = h < Bx ? 0 : fnx - 1;
+            const int r = h < Bx ? h : h - Bx;
+            k = 1 + r / (fny - 2);
+            j = 1 + (r - (k - 1) * (fny - 2));
+        }
+    }
+    const int src = bc_map(i, fnx, bcx) + bc_map(j, fny, bcy) * fnx + bc_map(k, fnz, bcz) * A + s * L;
+    const int dst = i + j * fnx + k * A + s * L;
+    ph[dst] = ph[src];
+    arg[dst] = arg[src];
+}
+
+// APTcollect_PF (3DgrainMPI/APTPhaseField.cu:55-75): argmax over slots with a
+// strict > from slot 0; written only if the max exceeds LS (double compare).
+__global__ __launch_bounds__(kBlock) void collect_kernel(const float* __restrict__ ph,
+                                                         const int* __restrict__ arg,
+                                                         float* __restrict__ phi,
+                                                         int* __restrict__ alpha, int L, int K,
+                                                         int clear_alpha)
+{
+    const int C = blockIdx.x * kBlock + threadIdx.x;
+    if (C >= L) return;
+    int am = 0;
+    float mv = ph[C];
+    for (int s = 1; s < K; ++s) {
+        const float v = ph[C + s * L];
+        if (v > mv) {
+            mv = v;
+            am = s;
+        }
+    }
+    if ((double)mv > kLS) {
+        phi[C] = mv;
+        alpha[C] = arg[C + am * L];
+    } else if (clear_alpha) {
+        alpha[C] = 0;
+    }
+}
+
+// set_minus1 + memset(-1) + APTini_PF on both buffers
+// (3DgrainMPI/APTPhaseField.cu:78-89,727-728,817-818).
+__global__ __launch_bounds__(kBlock) void ini_kernel(float* ph0, int* arg0, float* ph1, int* arg1,
+                                                     const float* __restrict__ phi0,
+                                                     const int* __restrict__ alpha0, int L, int K)
+{
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= L * K) return;
+    float v = -1.0f;
+    int id = -1;
+    if (q < L && (double)phi0[q] > kLS) {
+        v = phi0[q];
+        id = alpha0[q];
+    }
+    ph0[q] = v;
+    arg0[q] = id;
+    ph1[q] = v;
+    arg1[q] = id;
+}
+
+// ave_x (3DgrainMPI/PhaseField.cu:366-378) as an exact fixed-point sum:
+// every phi is rounded to a multiple of 2^-40 and summed in int64, which is
+// independent of summation order (the reference's float atomics are not).
+__global__ __launch_bounds__(kBlock) void plane_sum_kernel(const float* __restrict__ ph,
+                                                           long long* sums, int A, int L, int K)
+{
+    __shared__ long long red[kBlock];
+    const int k = blockIdx.y;
+    long long acc = 0;
+    for (int s = 0; s < K; ++s)
+        for (int q = blockIdx.x * kBlock + threadIdx.x; q < A; q += gridDim.x * kBlock)
+            acc += (long long)rint((double)ph[s * L + k * A + q] * 1099511627776.0);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd((unsigned long long*)&sums[k], (unsigned long long)red[0]);
+}
+
+// APTmove_frame, first half (3DgrainMPI/APTPhaseField.cu:357-391): shift every
+// slot of the interior down one plane into the other buffer (top plane
+// becomes -1/-1), shifted z into zbuf, archive alpha plane 1, count loss area.
+__global__ __launch_bounds__(kBlock) void move_frame_kernel(
+    const float* __restrict__ ph, const int* __restrict__ arg, float* __restrict__ phb,
+    int* __restrict__ argb, const float* __restrict__ z, float* __restrict__ zb,
+    const int* __restrict__ alpha, int* __restrict__ alpha_full, int* __restrict__ loss,
+    int move_count, int fnx, int fny, int fnz, int L, int K, int num_theta)
+{
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= (long long)L * K) return;
+    const int s = (int)(q / L);
+    const int C = (int)(q - (long long)s * L);
+    const int A = fnx * fny;
+    const int k = C / A;
+    const int r = C - k * A;
+    const int j = r / fnx;
+    const int i = r - j * fnx;
+    if (i == 0 && j == 0 && s == 0) {
+        if (k > 0 && k < fnz - 2) zb[k] = z[k + 1];
+        if (k == fnz - 2) zb[k] = 2 * z[fnz - 2] - z[fnz - 3];
+    }
+    if (i > 0 && i < fnx - 1 && j > 0 && j < fny - 1) {
+        if (k > 0 && k < fnz - 2) {
+            phb[q] = ph[q + A];
+            argb[q] = arg[q + A];
+        } else if (k == fnz - 2) {
+            phb[q] = -1.0f;
+            argb[q] = -1;
+        }
+    }
+    if (k == 1 && s == 0) {
+        const int al = alpha[C];
+        alpha_full[(long long)move_count * A + C] = al;
+        if (al >= 1 && al <= num_theta) atomicAdd(loss + al - 1, 1);
+    }
+}
+
+// APTcopy_frame (3DgrainMPI/APTPhaseField.cu:393-410).
+__global__ __launch_bounds__(kBlock) void copy_frame_kernel(
+    const float* __restrict__ phb, const int* __restrict__ argb, const float* __restrict__ zb,
+    float* __restrict__ ph, int* __restrict__ arg, float* __restrict__ z, int fnx, int fny,
+    int fnz, int L, int K)
+{
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= (long long)L * K) return;
+    const int s = (int)(q / L);
+    const int C = (int)(q - (long long)s * L);
+    const int A = fnx * fny;
+    const int k = C / A;
+    const int r = C - k * A;
+    const int j = r / fnx;
+    const int i = r - j * fnx;
+    if (i == 0 && j == 0 && s == 0 && k > 0 && k < fnz - 1) z[k] = zb[k];
+    if (i > 0 && i < fnx - 1 && j > 0 && j < fny - 1 && k > 0 && k < fnz - 1) {
+        ph[q] = phb[q];
+        arg[q] = argb[q];
+    }
+}
+
+// init_nucl_status (3DgrainMPI/APTPhaseField.cu:420-444), block centre taken
+// without the halo offset exactly as the reference does.
+__global__ __launch_bounds__(kBlock) void nucl_status_kernel(const float* __restrict__ phi0,
+                                                             int* status, int cnx, int cny,
+                                                             int cnz, int p, int fnx, int fny)
+{
+    const int C = blockIdx.x * kBlock + threadIdx.x;
+    if (C >= cnx * cny * cnz) return;
+    const int k = C / (cnx * cny);
+    const int r = C - k * cnx * cny;
+    const int j = r / cnx;
+    const int i = r - j * cnx;
+    const int w = 2 * p + 1;
+    const int g = (w * k + p) * fnx * fny + (w * j + p) * fnx + (w * i + p);
+    status[C] = ((double)phi0[g] > kLS) ? 1 : 0;
+}
+
+// splitmix64 finaliser (Steele, Lea & Flood, OOPSLA 2014).
+__device__ __forceinline__ uint64_t mix64(uint64_t v)
+{
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebULL;
+    return v ^ (v >> 31);
+}
+
+// nuncl_possibility (3DgrainMPI/APTPhaseField.cu:446-455).
+__device__ __forceinline__ float nucl_prob(const NuclArgs& a, float delT, float d_delT, float nmax)
+{
+    float slope = -0.5f * (delT - a.undcool_mean) * (delT - a.undcool_mean) / a.undcool_std
+                / a.undcool_std;
+    slope = expf(slope);
+    float density = nmax / (sqrtf((float)(2.0f * M_PI)) * a.undcool_std) * slope * d_delT;
+    return 8.0f * a.nuc_rad * a.nuc_rad * a.nuc_rad * density;
+}
+
+// add_nucl (3DgrainMPI/APTPhaseField.cu:459-536) with a counter-based draw per
+// (seed, coarse cell, call) instead of per-cell cuRAND state.
+__global__ __launch_bounds__(kBlock) void nucl_kernel(NuclArgs a)
+{
+    const int C = blockIdx.x * kBlock + threadIdx.x;
+    if (C >= a.cnx * a.cny * a.cnz) return;
+    if (a.status[C] != 0) return;
+    const int k = C / (a.cnx * a.cny);
+    const int r = C - k * a.cnx * a.cny;
+    const int j = r / a.cnx;
+    const int i = r - j * a.cnx;
+    const int w = 2 * a.p + 1;
+    const int gi = w * i + a.p + 1, gj = w * j + a.p + 1, gk = w * k + a.p + 1;
+    const int A = a.fnx * a.fny;
+    const int gC = gk * A + gj * a.fnx + gi;
+    int st = 0;
+    for (int s = 0; s < a.K; ++s)
+        if ((double)a.ph[gC + s * a.L] > kLS) st = 1;
+    if (st) {
+        a.status[C] = 1;
+        return;
+    }
+    float posb;
+    if (a.use_iuc) {
+        float delT = a.uc0 * 1e6 * (a.t + a.dt);
+        float d_delT = a.uc0 * 1e6 * a.dt;
+        posb = nucl_prob(a, delT, d_delT, a.nuc_Nmax0);
+    } else {
+        const float xx = a.x[gi] - a.X0_mac, yy = a.y[gj] - a.Y0_mac, zz = a.z[gk] - a.Z0_mac;
+        float Tc = interp4(a.T3d, xx, yy, zz, a.t - a.t0_mac, a.Nx, a.Ny, a.Nz, a.Nt, a.Dx_mac,
+                           a.Dt_mac);
+        float Tcd = interp4(a.T3d, xx, yy, zz, a.t + a.dt - a.t0_mac, a.Nx, a.Ny, a.Nz, a.Nt,
+                            a.Dx_mac, a.Dt_mac);
+        posb = nucl_prob(a, -Tcd, Tc - Tcd, a.nuc_Nmax);
+    }
+    uint64_t h = mix64(a.seed + 0x9e3779b97f4a7c15ULL);
+    h = mix64(h ^ ((uint64_t)C * 0xd1b54a32d192ed03ULL));
+    h = mix64(h ^ (a.call * 0x8cb92ba72f3d8dd7ULL + 0x632be59bd9b4e019ULL));
+    const float u = (float)((h >> 40) + 1) * (1.0f / 16777216.0f);
+    const int id = 1 + (int)(((h & 0xffffffffULL) * 0x7ffffffeULL) >> 32);
+    if (u < posb) {
+        int q = 0;
+        for (int lk = -a.p; lk <= a.p; ++lk)
+            for (int lj = -a.p; lj <= a.p; ++lj)
+                for (int li = -a.p; li <= a.p; ++li) {
+                    const int o = gC + lk * A + lj * a.fnx + li;
+                    const float v = a.ball[q++];
+                    a.ph[o] = v;
+                    if (v > -0.9999f) a.arg[o] = id;
+                }
+        a.status[C] = 1;
+        atomicAdd(a.events, 1ull);
+    }
+}
+
+inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream)
+{
+    const int nblocks = a.nblk_col * a.nzchunk;
+    dim3 grid(nblocks), block(kBlock), fgrid(kFixupBlocks);
+#define CUPF_LAUNCH(KK, GG)                                                                   \
+    do {                                                                                      \
+        hipLaunchKernelGGL((apt_step<KK, GG>), grid, block, 0, stream, a);                    \
+        hipLaunchKernelGGL((apt_fixup<KK, GG>), fgrid, block, 0, stream, a);                  \
+    } while (0)
+    if (!gather27) {
+        if (K == 5) CUPF_LAUNCH(5, false);
+        else if (K == 8) CUPF_LAUNCH(8, false);
+        else return hipErrorInvalidValue;
+    } else {
+        if (K == 5) CUPF_LAUNCH(5, true);
+        else if (K == 8) CUPF_LAUNCH(8, true);
+        else return hipErrorInvalidValue;
+    }
+#undef CUPF_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_bc_fill(float* ph, int* arg, int fnx, int fny, int fnz, int K, int bcx, int bcy,
+                          int bcz, hipStream_t stream)
+{
+    const int nhalo = 2 * fnx * fny + 2 * fnx * (fnz - 2) + 2 * (fny - 2) * (fnz - 2);
+    dim3 grid(cdiv(nhalo, kBlock), K);
+    hipLaunchKernelGGL(bc_fill, grid, dim3(kBlock), 0, stream, ph, arg, fnx, fny, fnz,
+                       fnx * fny * fnz, bcx, bcy, bcz, nhalo);
+    return hipGetLastError();
+}
+
+hipError_t launch_collect(const float* ph, const int* arg, float* phi, int* alpha, int L, int K,
+                          int clear_alpha, hipStream_t stream)
+{
+    hipLaunchKernelGGL(collect_kernel, dim3(cdiv(L, kBlock)), dim3(kBlock), 0, stream, ph, arg,
+                       phi, alpha, L, K, clear_alpha);
+    return hipGetLastError();
+}
+
+hipError_t launch_ini(float* ph0, int* arg0, float* ph1, int* arg1, const float* phi0,
+                      const int* alpha0, int L, int K, hipStream_t stream)
+{
+    hipLaunchKernelGGL(ini_kernel, dim3(cdiv((long long)L * K, kBlock)), dim3(kBlock), 0, stream,
+                       ph0, arg0, ph1, arg1, phi0, alpha0, L, K);
+    return hipGetLastError();
+}
+
+hipError_t launch_plane_sums(const float* ph, long long* sums, int fnx, int fny, int fnz, int K,
+                             hipStream_t stream)
+{
+    const int A = fnx * fny;
+    hipError_t e = hipMemsetAsync(sums, 0, sizeof(long long) * fnz, stream);
+    if (e != hipSuccess) return e;
+    dim3 grid(min(cdiv(A, kBlock), 64), fnz);
+    hipLaunchKernelGGL(plane_sum_kernel, grid, dim3(kBlock), 0, stream, ph, sums, A, A * fnz, K);
+    return hipGetLastError();
+}
+
+hipError_t launch_move_frame(float* ph_old, int* arg_old, float* ph_new, int* arg_new, float* z,
+                             float* zbuf, const int* alpha, int* alpha_full, int* loss_area,
+                             int move_count, int fnx, int fny, int fnz, int K, int num_theta,
+                             hipStream_t stream)
+{
+    const int L = fnx * fny * fnz;
+    dim3 grid(cdiv((long long)L * K, kBlock));
+    hipLaunchKernelGGL(move_frame_kernel, grid, dim3(kBlock), 0, stream, ph_old, arg_old, ph_new,
+                       arg_new, z, zbuf, alpha, alpha_full, loss_area, move_count, fnx, fny, fnz,
+                       L, K, num_theta);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(copy_frame_kernel, grid, dim3(kBlock), 0, stream, ph_new, arg_new, zbuf,
+                       ph_old, arg_old, z, fnx, fny, fnz, L, K);
+    return hipGetLastError();
+}
+
+hipError_t launch_nucl_status(const float* phi0, int* status, int cnx, int cny, int cnz, int p,
+                              int fnx, int fny, hipStream_t stream)
+{
+    const int n = cnx * cny * cnz;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(nucl_status_kernel, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, stream, phi0,
+                       status, cnx, cny, cnz, p, fnx, fny);
+    return hipGetLastError();
+}
+
+hipError_t launch_nucl(const NuclArgs& a, hipStream_t stream)
+{
+    const int n = a.cnx * a.cny * a.cnz;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(nucl_kernel, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace cupf

@@ -1,0 +1,661 @@
+// pf_capi.hip -- C ABI of cupf_amd (declared in include/pf_capi.h).
+//
+// Owns the device state of one simulation: the two ping-pong K-slot buffers
+// (phi float32 + grain id int32), coordinates, orientation and thermal tables,
+// single-field phi/alpha, moving-domain archive and nucleation state.  All
+// work goes to the context's own HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pf_capi.h"
+#include "apt_internal.h"
+
+using namespace cupf;
+
+struct pf_ctx {
+    pf_params p;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int L = 0;
+    int cur = 0;                    // index of the current buffer
+    float* ph[2] = {nullptr, nullptr};
+    int* arg[2] = {nullptr, nullptr};
+    float *x = nullptr, *y = nullptr, *z = nullptr, *zbuf = nullptr;
+    float* R12 = nullptr;
+    float* psi_mac = nullptr;
+    float* T3d = nullptr;
+    float* phi0 = nullptr;          // initial single field (phi_old of the reference)
+    float* phi = nullptr;           // collect target
+    int* alpha = nullptr;           // collect target / initial alpha (alpha_m)
+    unsigned long long* counters = nullptr;  // [0] overflow points, [1] nucleation events
+    unsigned* fix_list = nullptr;   // overflow fix-up list, one entry per interior point
+    unsigned* fix_count = nullptr;  // [0] entries, [1] finished fix-up workgroups
+    long long* sums = nullptr;
+    std::vector<long long> h_sums;
+    int* alpha_full = nullptr;
+    int* loss = nullptr;
+    int fnz_full = 0;
+    int* nstatus = nullptr;
+    float* ball = nullptr;
+    int cnx = 0, cny = 0, cnz = 0;
+    unsigned long long ncall = 0;
+    bool have_coords = false, have_orient = false, have_state = false;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(PF_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+template <class T>
+int dalloc(T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) return PF_OK;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(PF_E_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) +
+                                    " bytes failed: " + hipGetErrorString(e));
+    }
+    return PF_OK;
+}
+
+int ctx_check(pf_ctx* c)
+{
+    if (!c) return fail(PF_E_INVALID, "null context");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(PF_E_HIP, "hipSetDevice failed");
+    return PF_OK;
+}
+
+int buf_index(pf_ctx* c, int which) { return which == 0 ? c->cur : 1 - c->cur; }
+
+StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
+{
+    const pf_params& p = c->p;
+    StepArgs a;
+    a.ph = c->ph[src];
+    a.arg = c->arg[src];
+    a.ph_new = c->ph[1 - src];
+    a.arg_new = c->arg[1 - src];
+    a.x = c->x;
+    a.y = c->y;
+    a.z = c->z;
+    a.R12 = c->R12;
+    a.psi_mac = c->psi_mac;
+    a.overflow = c->counters;
+    a.fix_list = c->fix_list;
+    a.fix_count = c->fix_count;
+    a.fnx = p.fnx;
+    a.fny = p.fny;
+    a.fnz = p.fnz;
+    a.L = c->L;
+    a.ncolx = p.fnx - 2;
+    a.ncols = (p.fnx - 2) * (p.fny - 2);
+    a.kchunk = 16;
+    a.nzchunk = (p.fnz - 2 + a.kchunk - 1) / a.kchunk;
+    a.nblk_col = (a.ncols + 255) / 256;
+    a.bcx = p.bcx;
+    a.bcy = p.bcy;
+    a.bcz = p.bcz;
+    a.num_theta = p.num_theta;
+    a.thermal_type = p.thermal_type;
+    a.use_iuc = use_iuc;
+    a.t = t;
+    a.dt = p.dt;
+    a.hi = p.hi;
+    a.lamd = p.lamd;
+    a.L_cp = p.L_cp;
+    a.kin_delta = p.kin_delta;
+    a.eps = p.eps;
+    a.G = p.G;
+    a.R = p.R;
+    a.uc = p.uc;
+    a.uc0 = p.uc0;
+    a.z0 = p.z0;
+    a.r0 = p.r0;
+    a.lyd = p.lyd;
+    a.lzd = p.lzd;
+    a.V = p.V;
+    a.mp_len = p.mp_len;
+    a.x0 = p.x0;
+    a.order = p.order;
+    a.tan_angle = p.tan_angle;
+    a.cos_angle = p.cos_angle;
+    a.Nx = p.Nx;
+    a.Ny = p.Ny;
+    a.Nz = p.Nz;
+    a.X0_mac = p.X0_mac;
+    a.Y0_mac = p.Y0_mac;
+    a.Z0_mac = p.Z0_mac;
+    a.Dx_mac = p.Dx_mac;
+    return a;
+}
+
+int nucl_launch(pf_ctx* c, float t, float dt_nuc, int use_iuc)
+{
+    if (!c->nstatus) return fail(PF_E_STATE, "nucleation requested before pf_nucl_reset");
+    const pf_params& p = c->p;
+    NuclArgs a;
+    a.ph = c->ph[c->cur];
+    a.arg = c->arg[c->cur];
+    a.status = c->nstatus;
+    a.x = c->x;
+    a.y = c->y;
+    a.z = c->z;
+    a.T3d = c->T3d;
+    a.ball = c->ball;
+    a.events = c->counters + 1;
+    a.fnx = p.fnx;
+    a.fny = p.fny;
+    a.fnz = p.fnz;
+    a.L = c->L;
+    a.K = p.K;
+    a.cnx = c->cnx;
+    a.cny = c->cny;
+    a.cnz = c->cnz;
+    a.p = p.pts_cell;
+    a.use_iuc = use_iuc;
+    a.t = t;
+    a.dt = dt_nuc;
+    a.uc0 = p.uc0;
+    a.undcool_mean = p.undcool_mean;
+    a.undcool_std = p.undcool_std;
+    a.nuc_Nmax = p.nuc_Nmax;
+    a.nuc_Nmax0 = p.nuc_Nmax0;
+    a.nuc_rad = p.nuc_rad;
+    a.Nx = p.Nx;
+    a.Ny = p.Ny;
+    a.Nz = p.Nz;
+    a.Nt = p.Nt;
+    a.X0_mac = p.X0_mac;
+    a.Y0_mac = p.Y0_mac;
+    a.Z0_mac = p.Z0_mac;
+    a.t0_mac = p.t0_mac;
+    a.Dx_mac = p.Dx_mac;
+    a.Dt_mac = p.Dt_mac;
+    a.seed = p.seed;
+    a.call = c->ncall++;
+    if (!use_iuc && !c->T3d)
+        return fail(PF_E_STATE, "nucleation needs the T3d macro field (pf_set_thermal_macro)");
+    HIPCHK(launch_nucl(a, c->stream));
+    return PF_OK;
+}
+
+void free_ctx(pf_ctx* c)
+{
+    (void)hipSetDevice(c->device);
+    void* bufs[] = {c->ph[0], c->ph[1], c->arg[0], c->arg[1], c->x, c->y, c->z, c->zbuf,
+                    c->R12, c->psi_mac, c->T3d, c->phi0, c->phi, c->alpha, c->counters,
+                    c->sums, c->alpha_full, c->loss, c->nstatus, c->ball,
+                    c->fix_list, c->fix_count};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pf_last_error(void) { return g_err.c_str(); }
+
+const char* pf_build_info(void) { return "cupf_amd gfx950 K={5,8} gather={7,27}"; }
+
+int pf_create(const pf_params* params, int device, pf_ctx** out)
+{
+    if (!params || !out) return fail(PF_E_INVALID, "null argument");
+    *out = nullptr;
+    const pf_params& p = *params;
+    if (p.K != 5 && p.K != 8) return fail(PF_E_INVALID, "K must be 5 or 8");
+    if (p.fnx < 4 || p.fny < 4 || p.fnz < 4)
+        return fail(PF_E_INVALID, "grid too small: fnx, fny, fnz must be >= 4");
+    if (p.num_theta < 1) return fail(PF_E_INVALID, "num_theta must be >= 1");
+    if (p.bcx < 0 || p.bcx > 1 || p.bcy < 0 || p.bcy > 1 || p.bcz < 0 || p.bcz > 1)
+        return fail(PF_E_INVALID, "bc digits must be 0 or 1");
+    if (p.thermal_type < 0 || p.thermal_type > 5)
+        return fail(PF_E_INVALID, "thermal_type must be 0..5");
+    const long long L = (long long)p.fnx * p.fny * p.fnz;
+    if (L * p.K >= (1LL << 31))
+        return fail(PF_E_INVALID, "K*fnx*fny*fnz must stay below 2^31 (32-bit slot indexing)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(PF_E_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(PF_E_INVALID, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+
+    pf_ctx* c = new pf_ctx();
+    c->p = p;
+    c->device = device;
+    c->L = (int)L;
+    int rc = PF_OK;
+    auto bail = [&](int code) {
+        free_ctx(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(PF_E_HIP, "hipStreamCreate failed"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return bail(fail(PF_E_HIP, "hipEventCreate failed"));
+    const size_t nst = (size_t)L * p.K;
+    for (int b = 0; b < 2; ++b) {
+        if ((rc = dalloc(&c->ph[b], nst)) || (rc = dalloc(&c->arg[b], nst))) return bail(rc);
+    }
+    if ((rc = dalloc(&c->x, p.fnx)) || (rc = dalloc(&c->y, p.fny)) || (rc = dalloc(&c->z, p.fnz)) ||
+        (rc = dalloc(&c->zbuf, p.fnz)) || (rc = dalloc(&c->phi0, (size_t)L)) ||
+        (rc = dalloc(&c->phi, (size_t)L)) || (rc = dalloc(&c->alpha, (size_t)L)) ||
+        (rc = dalloc(&c->counters, 2)) || (rc = dalloc(&c->sums, p.fnz)) ||
+        (rc = dalloc(&c->fix_list, (size_t)(p.fnx - 2) * (p.fny - 2) * (p.fnz - 2))) ||
+        (rc = dalloc(&c->fix_count, 2)))
+        return bail(rc);
+    c->h_sums.resize(p.fnz);
+    for (int b = 0; b < 2; ++b) {
+        if (hipMemsetD32Async((hipDeviceptr_t)c->ph[b], 0xBF800000u, nst, c->stream) != hipSuccess ||
+            hipMemsetAsync(c->arg[b], 0xFF, nst * sizeof(int), c->stream) != hipSuccess)
+            return bail(fail(PF_E_HIP, "initial fill failed"));
+    }
+    if (hipMemsetAsync(c->counters, 0, 2 * sizeof(unsigned long long), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->fix_count, 0, 2 * sizeof(unsigned), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->alpha, 0, L * sizeof(int), c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return bail(fail(PF_E_HIP, "initial fill failed"));
+    *out = c;
+    return PF_OK;
+}
+
+int pf_destroy(pf_ctx* ctx)
+{
+    if (!ctx) return PF_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    free_ctx(ctx);
+    return PF_OK;
+}
+
+int pf_set_coords(pf_ctx* c, const float* x, const float* y, const float* z)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!x || !y || !z) return fail(PF_E_INVALID, "null coordinate array");
+    HIPCHK(hipMemcpyAsync(c->x, x, sizeof(float) * c->p.fnx, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->y, y, sizeof(float) * c->p.fny, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->z, z, sizeof(float) * c->p.fnz, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_coords = true;
+    return PF_OK;
+}
+
+int pf_orientations_from_angles(const float* theta, int ng, float* R9)
+{
+    if (!theta || !R9 || ng < 1) return fail(PF_E_INVALID, "bad arguments");
+    for (int g = 0; g <= ng; ++g) {
+        // sint/cost tables of 3DgrainMPI/PhaseFieldCpu.cpp:379-385, indexed as
+        // in 3DgrainMPI/APTPhaseField.cu:262-265 (row 0 pairs theta[0] with theta[NG]).
+        const float sina = sinf(theta[g]), cosa = cosf(theta[g]);
+        const float sinb = sinf(theta[g + ng]), cosb = cosf(theta[g + ng]);
+        float* m = R9 + 9 * (size_t)g;
+        m[0] = cosa * cosb;
+        m[1] = sina * cosb;
+        m[2] = -sinb;
+        m[3] = -sina;
+        m[4] = cosa;
+        m[5] = 0.0f;
+        m[6] = cosa * sinb;
+        m[7] = sina * sinb;
+        m[8] = cosb;
+    }
+    return PF_OK;
+}
+
+int pf_set_orientations(pf_ctx* c, const float* R9, int nrows)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!R9) return fail(PF_E_INVALID, "null orientation table");
+    if (nrows < c->p.num_theta + 1)
+        return fail(PF_E_INVALID, "orientation table needs num_theta+1 rows");
+    std::vector<float> r12((size_t)nrows * 12, 0.0f);
+    for (int g = 0; g < nrows; ++g)
+        for (int q = 0; q < 9; ++q) r12[12 * (size_t)g + q] = R9[9 * (size_t)g + q];
+    if (c->R12) (void)hipFree(c->R12);
+    c->R12 = nullptr;
+    if ((rc = dalloc(&c->R12, r12.size()))) return rc;
+    HIPCHK(hipMemcpyAsync(c->R12, r12.data(), sizeof(float) * r12.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_orient = true;
+    return PF_OK;
+}
+
+int pf_set_thermal_macro(pf_ctx* c, const float* psi_mac, const float* T3d)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    const pf_params& p = c->p;
+    const size_t n3 = (size_t)p.Nx * p.Ny * p.Nz;
+    if ((psi_mac || T3d) && (p.Nx < 2 || p.Ny < 2 || p.Nz < 2))
+        return fail(PF_E_INVALID, "macro grid needs Nx, Ny, Nz >= 2");
+    if (psi_mac) {
+        if (c->psi_mac) (void)hipFree(c->psi_mac);
+        c->psi_mac = nullptr;
+        if ((rc = dalloc(&c->psi_mac, n3))) return rc;
+        HIPCHK(hipMemcpyAsync(c->psi_mac, psi_mac, sizeof(float) * n3, hipMemcpyHostToDevice,
+                              c->stream));
+    }
+    if (T3d) {
+        if (p.Nt < 2) return fail(PF_E_INVALID, "T3d needs Nt >= 2");
+        if (c->T3d) (void)hipFree(c->T3d);
+        c->T3d = nullptr;
+        if ((rc = dalloc(&c->T3d, n3 * p.Nt))) return rc;
+        HIPCHK(hipMemcpyAsync(c->T3d, T3d, sizeof(float) * n3 * p.Nt, hipMemcpyHostToDevice,
+                              c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_init_from_phi_alpha(pf_ctx* c, const float* phi0, const int* alpha0)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!phi0 || !alpha0) return fail(PF_E_INVALID, "null initial field");
+    const size_t L = c->L;
+    HIPCHK(hipMemcpyAsync(c->phi0, phi0, sizeof(float) * L, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->phi, phi0, sizeof(float) * L, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->alpha, alpha0, sizeof(int) * L, hipMemcpyHostToDevice, c->stream));
+    c->cur = 0;
+    HIPCHK(launch_ini(c->ph[0], c->arg[0], c->ph[1], c->arg[1], c->phi0, c->alpha, c->L, c->p.K,
+                      c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return PF_OK;
+}
+
+int pf_reinit_from_collect(pf_ctx* c)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!c->have_state) return fail(PF_E_STATE, "no initial field");
+    c->cur = 0;
+    HIPCHK(launch_ini(c->ph[0], c->arg[0], c->ph[1], c->arg[1], c->phi0, c->alpha, c->L, c->p.K,
+                      c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_upload_state(pf_ctx* c, int which, const float* ph, const int* arg)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!ph || !arg) return fail(PF_E_INVALID, "null state array");
+    const int b = buf_index(c, which);
+    const size_t n = (size_t)c->L * c->p.K;
+    HIPCHK(hipMemcpyAsync(c->ph[b], ph, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->arg[b], arg, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return PF_OK;
+}
+
+int pf_download_state(pf_ctx* c, int which, float* ph, int* arg)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    const int b = buf_index(c, which);
+    const size_t n = (size_t)c->L * c->p.K;
+    if (ph) HIPCHK(hipMemcpyAsync(ph, c->ph[b], sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    if (arg) HIPCHK(hipMemcpyAsync(arg, c->arg[b], sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_apply_bc(pf_ctx* c, int which)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    const int b = buf_index(c, which);
+    const pf_params& p = c->p;
+    HIPCHK(launch_bc_fill(c->ph[b], c->arg[b], p.fnx, p.fny, p.fnz, p.K, p.bcx, p.bcy, p.bcz,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+static int step_impl(pf_ctx* c, int nsteps, int first_step, int flags)
+{
+    if (!c->have_coords || !c->have_orient || !c->have_state)
+        return fail(PF_E_STATE, "pf_step needs coordinates, orientations and an initial state");
+    const pf_params& p = c->p;
+    if (p.thermal_type == PF_THERMAL_PARABOLOID && !c->psi_mac)
+        return fail(PF_E_STATE, "paraboloid thermal field needs psi_mac");
+    if (nsteps < 0) return fail(PF_E_INVALID, "nsteps must be >= 0");
+    const int use_iuc = (flags & PF_STEP_INITIAL_UC) ? 1 : 0;
+    const bool nucl = (flags & PF_STEP_NUCLEATE) != 0;
+    for (int s = first_step; s < first_step + nsteps; ++s) {
+        // t_cur_step = (2*kt+1)*dt*tau0 in float (3DgrainMPI/APTPhaseField.cu:863,919).
+        const float t = s * p.dt * p.tau0;
+        StepArgs a = make_step_args(c, c->cur, t, use_iuc);
+        HIPCHK(launch_step(a, p.K, p.gather27, c->stream));
+        c->cur = 1 - c->cur;
+        if (nucl && (s & 1)) {
+            int rc = nucl_launch(c, t, 2.0f * p.dt * p.tau0, use_iuc);
+            if (rc) return rc;
+        }
+    }
+    return PF_OK;
+}
+
+int pf_step(pf_ctx* c, int nsteps, int first_step, int flags)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    return step_impl(c, nsteps, first_step, flags);
+}
+
+int pf_step_timed(pf_ctx* c, int nsteps, int first_step, int flags, float* ms)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    rc = step_impl(c, nsteps, first_step, flags);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    if (ms) *ms = t;
+    return PF_OK;
+}
+
+int pf_collect(pf_ctx* c, int which, int flags, float* phi_out, int* alpha_out)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    const int b = buf_index(c, which);
+    const pf_params& p = c->p;
+    if (!(flags & PF_COLLECT_NO_BC))
+        HIPCHK(launch_bc_fill(c->ph[b], c->arg[b], p.fnx, p.fny, p.fnz, p.K, p.bcx, p.bcy, p.bcz,
+                              c->stream));
+    HIPCHK(launch_collect(c->ph[b], c->arg[b], c->phi, c->alpha, c->L, p.K,
+                          (flags & PF_COLLECT_CLEAR_ALPHA) ? 1 : 0, c->stream));
+    if (phi_out)
+        HIPCHK(hipMemcpyAsync(phi_out, c->phi, sizeof(float) * c->L, hipMemcpyDeviceToHost, c->stream));
+    if (alpha_out)
+        HIPCHK(hipMemcpyAsync(alpha_out, c->alpha, sizeof(int) * c->L, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_md_enable(pf_ctx* c, int fnz_full)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (fnz_full < c->p.fnz) return fail(PF_E_INVALID, "fnz_full must be >= fnz");
+    if (c->alpha_full) (void)hipFree(c->alpha_full);
+    if (c->loss) (void)hipFree(c->loss);
+    c->alpha_full = nullptr;
+    c->loss = nullptr;
+    const size_t nf = (size_t)c->p.fnx * c->p.fny * fnz_full;
+    if ((rc = dalloc(&c->alpha_full, nf)) || (rc = dalloc(&c->loss, c->p.num_theta))) return rc;
+    HIPCHK(hipMemsetAsync(c->alpha_full, 0, nf * sizeof(int), c->stream));
+    HIPCHK(hipMemsetAsync(c->loss, 0, c->p.num_theta * sizeof(int), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->fnz_full = fnz_full;
+    return PF_OK;
+}
+
+int pf_md_tip(pf_ctx* c, int* tip)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!tip) return fail(PF_E_INVALID, "null tip");
+    const pf_params& p = c->p;
+    const int b = c->cur;
+    HIPCHK(launch_bc_fill(c->ph[b], c->arg[b], p.fnx, p.fny, p.fnz, p.K, p.bcx, p.bcy, p.bcz,
+                          c->stream));
+    HIPCHK(launch_plane_sums(c->ph[b], c->sums, p.fnx, p.fny, p.fnz, p.K, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_sums.data(), c->sums, sizeof(long long) * p.fnz,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // tip_mvf scan (3DgrainMPI/PhaseField.cu:391-394) on the exact plane sums.
+    const double denom = (double)p.K * p.fnx * p.fny;
+    int t = *tip;
+    if (t < 0 || t >= p.fnz) return fail(PF_E_INVALID, "tip out of range");
+    while (std::ldexp((double)c->h_sums[t], -40) / denom > kLS && t < p.fnz - 1) ++t;
+    *tip = t;
+    return PF_OK;
+}
+
+int pf_md_shift(pf_ctx* c, int move_count)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!c->alpha_full) return fail(PF_E_STATE, "pf_md_enable first");
+    const pf_params& p = c->p;
+    const size_t A = (size_t)p.fnx * p.fny;
+    if ((size_t)(move_count + 2) * A > (size_t)c->fnz_full * A)
+        return fail(PF_E_INVALID, "alpha_full archive is full");
+    const int b = c->cur;
+    // APTcollect_PF on the buffer as stored (3DgrainMPI/APTPhaseField.cu:683),
+    // then APTmove_frame with the other buffer as scratch, then APTcopy_frame.
+    HIPCHK(launch_collect(c->ph[b], c->arg[b], c->phi, c->alpha, c->L, p.K, 0, c->stream));
+    HIPCHK(launch_move_frame(c->ph[b], c->arg[b], c->ph[1 - b], c->arg[1 - b], c->z, c->zbuf,
+                             c->alpha, c->alpha_full, c->loss, move_count, p.fnx, p.fny, p.fnz,
+                             p.K, p.num_theta, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_md_download(pf_ctx* c, int* alpha_full, int* loss_area, float* z)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    const pf_params& p = c->p;
+    if (alpha_full) {
+        if (!c->alpha_full) return fail(PF_E_STATE, "pf_md_enable first");
+        HIPCHK(hipMemcpyAsync(alpha_full, c->alpha_full,
+                              sizeof(int) * (size_t)p.fnx * p.fny * c->fnz_full,
+                              hipMemcpyDeviceToHost, c->stream));
+    }
+    if (loss_area) {
+        if (!c->loss) return fail(PF_E_STATE, "pf_md_enable first");
+        HIPCHK(hipMemcpyAsync(loss_area, c->loss, sizeof(int) * p.num_theta, hipMemcpyDeviceToHost,
+                              c->stream));
+    }
+    if (z) HIPCHK(hipMemcpyAsync(z, c->z, sizeof(float) * p.fnz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_nucl_reset(pf_ctx* c, int mode)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    const pf_params& p = c->p;
+    if (p.pts_cell < 0) return fail(PF_E_INVALID, "pts_cell must be >= 0");
+    const int w = 2 * p.pts_cell + 1;
+    // cnx = (fnx - 1 - 2*haloWidth)/(2*pts_cell+1) (3DgrainMPI/APTPhaseField.cu:735-737)
+    c->cnx = (p.fnx - 3) / w;
+    c->cny = (p.fny - 3) / w;
+    c->cnz = (p.fnz - 3) / w;
+    const size_t n = (size_t)c->cnx * c->cny * c->cnz;
+    if (c->nstatus) (void)hipFree(c->nstatus);
+    if (c->ball) (void)hipFree(c->ball);
+    c->nstatus = nullptr;
+    c->ball = nullptr;
+    if ((rc = dalloc(&c->nstatus, n > 0 ? n : 1)) || (rc = dalloc(&c->ball, (size_t)w * w * w)))
+        return rc;
+    // tanh ball of add_nucl (3DgrainMPI/APTPhaseField.cu:515-529), a function of
+    // the offset only, evaluated once on the host.
+    std::vector<float> ball((size_t)w * w * w);
+    int q = 0;
+    for (int lk = -p.pts_cell; lk <= p.pts_cell; ++lk)
+        for (int lj = -p.pts_cell; lj <= p.pts_cell; ++lj)
+            for (int li = -p.pts_cell; li <= p.pts_cell; ++li) {
+                const float pc = (float)p.pts_cell;
+                const float d = p.dx * ((1.0f + pc) / 2.0f
+                                        - sqrtf((float)(li * li + lj * lj + lk * lk)));
+                ball[q++] = tanhf(d / p.sqrt2);
+            }
+    HIPCHK(hipMemcpyAsync(c->ball, ball.data(), sizeof(float) * ball.size(),
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->nstatus, 0, sizeof(int) * (n > 0 ? n : 1), c->stream));
+    if (mode == 1)
+        HIPCHK(launch_nucl_status(c->phi0, c->nstatus, c->cnx, c->cny, c->cnz, p.pts_cell, p.fnx,
+                                  p.fny, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_nucl_events(pf_ctx* c, long long* events)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    unsigned long long v = 0;
+    HIPCHK(hipMemcpyAsync(&v, c->counters + 1, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (events) *events = (long long)v;
+    return PF_OK;
+}
+
+int pf_overflow_count(pf_ctx* c, long long* count)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    unsigned long long v = 0;
+    HIPCHK(hipMemcpyAsync(&v, c->counters, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (count) *count = (long long)v;
+    return PF_OK;
+}
+
+int pf_sync(pf_ctx* c)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+}  // extern "C"
