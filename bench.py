@@ -1,0 +1,182 @@
+"""Benchmark of the APT step on one MI355X.
+
+Workload: the 256x256x512 line-configuration simulation (K=8 active slots,
+~5000 seeded Voronoi grains, random quaternion orientations, linear thermal
+field G=1e6 K/m, R=0.1 m/s, BC 110), state resident in HBM.  A step is one
+APTrhs_psi time step over all (nx+1)(ny+1)(nz+1) updated points.
+
+Prints one JSON line:
+  value      Mcell-steps/s over the timed steps (whole job, one GPU)
+  roofline   the main step kernel's algorithmic HBM bytes (updated points x
+             K x 16 B: one read and one write of every slot's float32 phi and
+             int32 id) over its average HIP-event duration, against 8 TB/s
+  cpu_baseline  the CPU oracle (float32 restatement, OpenMP) on the same
+             post-warmup state for a few steps
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(workload: str):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get(workload, {}).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=400)
+    ap.add_argument("--K", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-only", type=int, default=0,
+                    help="only time a plain copy kernel over this many bytes (counter calibration)")
+    ap.add_argument("--z0-cells", type=float, default=None,
+                    help="initial interface height in cells (default: z0 = 2 um as in line.py)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        return 2
+
+    from cupf_amd import capi, cases
+
+    if args.copy_only:
+        gbs = capi.copy_bandwidth(args.copy_only, 5)
+        print(json.dumps({"copy_bytes": args.copy_only, "copy_gbs": round(gbs, 1)}), flush=True)
+        return 0
+
+    t0 = time.time()
+    case = cases.c2(K=args.K, z0_cells=args.z0_cells)
+    p = case.params
+    n_upd = (p["fnx"] - 2) * (p["fny"] - 2) * (p["fnz"] - 2)
+    log(f"[bench] case {case.name}: fnx,fny,fnz={p['fnx']},{p['fny']},{p['fnz']} grains={case.n_grains} "
+        f"K={args.K} updated points={n_upd} (setup {time.time() - t0:.1f}s)")
+
+    ctx = capi.APTContext(p)
+    ctx.set_coords(case.x, case.y, case.z)
+    ctx.set_orientations(case.R9)
+    ctx.init_from_phi_alpha(case.phi0, case.alpha0)
+
+    step0 = 1
+    if args.warmup:
+        ms = ctx.step_timed(args.warmup, step0)
+        log(f"[bench] warmup {args.warmup} steps: {ms / args.warmup:.3f} ms/step")
+        step0 += args.warmup
+    ovf0 = ctx.overflow_count()
+
+    # timed region: stream synchronised on both sides, wall clock + HIP events
+    ctx.sync()
+    w0 = time.perf_counter()
+    ev_ms = ctx.step_timed(args.steps, step0)
+    ctx.sync()
+    wall = time.perf_counter() - w0
+    step0 += args.steps
+    ovf = ctx.overflow_count() - ovf0
+    log(f"[bench] timed {args.steps} steps: wall {wall * 1e3 / args.steps:.3f} ms/step, "
+        f"events {ev_ms / args.steps:.3f} ms/step, overflow points {ovf}")
+
+    # per-kernel durations (HIP events around every launch on the context's stream)
+    nprof = min(args.steps, 100)
+    ms_main, ms_fix = ctx.step_profiled(nprof, step0)
+    step0 += nprof
+    avg_main = ms_main / nprof
+    log(f"[bench] per launch: apt_step {avg_main:.4f} ms, apt_fixup {ms_fix / nprof:.4f} ms")
+
+    copy_gbs = capi.copy_bandwidth(1 << 30, 10)
+    log(f"[bench] plain copy kernel: {copy_gbs:.0f} GB/s")
+    bytes_step = n_upd * args.K * 16
+    achieved = bytes_step / (avg_main * 1e-3) / 1e9
+    mcell = n_upd * args.steps / wall / 1e6
+
+    cpu = None
+    if not args.no_cpu and args.cpu_steps > 0:
+        import oracle
+        ph, arg = ctx.download_state(0)
+        sim = oracle.OracleSim(p, case.x, case.y, case.z, case.R9, "f32")
+        sim.set_state(ph, arg, 0)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        c0 = time.perf_counter()
+        sim.step(args.cpu_steps, step0)
+        csec = time.perf_counter() - c0
+        cpu = {
+            "value": round(n_upd * args.cpu_steps / csec / 1e6, 3),
+            "unit": "Mcell-steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{args.cpu_steps} steps of the same 256x256x512 K=8 state after "
+                      f"{args.warmup} GPU warmup steps (float32 oracle, OpenMP {threads} threads, "
+                      f"{csec:.1f} s)",
+        }
+        log(f"[bench] cpu oracle: {cpu['value']} Mcell-steps/s on {threads} threads")
+
+    out = {
+        "metric": "Mcell-steps/s + %HBM roofline at 256^2x512 K=8",
+        "value": round(mcell, 2),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+i32",
+        "data": "synthetic: seeded periodic Voronoi substrate, random quaternion orientations, "
+                "line thermal field; state resident in HBM",
+        "config": {
+            "workload": "line-256x256x512",
+            "grid": [p["fnx"], p["fny"], p["fnz"]],
+            "K": args.K,
+            "grains": case.n_grains,
+            "G_K_per_m": 1e6,
+            "R_m_per_s": 0.1,
+            "bc": "110",
+            "z0_um": float(p["z0"]),
+            "overflow_points_timed": ovf,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles("line-256x256x512"),
+            "kernel": "apt_step_tiled<8,false>",
+            "copy_kernel_gbs": round(copy_gbs, 1),
+            "frac_of_copy": round(achieved / copy_gbs, 4),
+            "kernel_ms": round(avg_main, 4),
+            "algorithmic_bytes_per_launch": bytes_step,
+        },
+        "cpu_baseline": cpu,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -100,6 +100,7 @@ def load() -> ctypes.CDLL:
         "pf_apply_bc": ([vp, c_int], c_int),
         "pf_step": ([vp, c_int, c_int, c_int], c_int),
         "pf_step_timed": ([vp, c_int, c_int, c_int, fp], c_int),
+        "pf_step_profiled": ([vp, c_int, c_int, c_int, fp, fp], c_int),
         "pf_collect": ([vp, c_int, c_int, fp, ip], c_int),
         "pf_md_enable": ([vp, c_int], c_int),
         "pf_md_tip": ([vp, ip], c_int),
@@ -109,6 +110,7 @@ def load() -> ctypes.CDLL:
         "pf_nucl_events": ([vp, llp], c_int),
         "pf_overflow_count": ([vp, llp], c_int),
         "pf_sync": ([vp], c_int),
+        "pf_copy_bandwidth": ([c_int, ctypes.c_ulonglong, c_int, fp], c_int),
     }
     for name, (argt, rest) in sig.items():
         f = getattr(lib, name)
@@ -138,6 +140,13 @@ def orientations_from_angles(theta: np.ndarray, num_theta: int) -> np.ndarray:
     out = np.zeros((num_theta + 1, 9), dtype=np.float32)
     _check(load().pf_orientations_from_angles(_fptr(th), num_theta, _fptr(out)))
     return out
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 10, device: int = 0) -> float:
+    """GB/s (read + write) of a plain copy kernel on the device."""
+    g = ctypes.c_float()
+    _check(load().pf_copy_bandwidth(device, nbytes, iters, ctypes.byref(g)))
+    return g.value
 
 
 class APTContext:
@@ -218,6 +227,13 @@ class APTContext:
         ms = ctypes.c_float()
         _check(load().pf_step_timed(self._h, nsteps, first_step, flags, ctypes.byref(ms)))
         return ms.value
+
+    def step_profiled(self, nsteps: int, first_step: int = 1, flags: int = 0):
+        """(ms in the main step kernel, ms in the fix-up kernel), summed over nsteps."""
+        m, f = ctypes.c_float(), ctypes.c_float()
+        _check(load().pf_step_profiled(self._h, nsteps, first_step, flags, ctypes.byref(m),
+                                       ctypes.byref(f)))
+        return m.value, f.value
 
     def sync(self):
         _check(load().pf_sync(self._h))

@@ -28,11 +28,15 @@ struct StepArgs {
     unsigned long long* overflow;  // count of points with > K distinct ids
     unsigned* fix_list;    // interior points left to the fix-up pass (capacity: interior points)
     unsigned* fix_count;   // [0] entries in fix_list, [1] fix-up workgroups finished
+    int* sink_i;           // 64-lane write sinks for lanes that must not store
+    float* sink_f;
     int fnx, fny, fnz;
     int L;                 // slot stride fnx*fny*fnz
     int ncolx, ncols;      // interior columns: (fnx-2), (fnx-2)*(fny-2)
     int nzchunk, kchunk;   // z-chunks per column, planes per chunk
-    int nblk_col;          // column blocks
+    int nblk_col;          // column blocks (direct path)
+    int tiles_x, tiles_y;  // full 64x4 column tiles (LDS path)
+    int edge_blocks;       // 256-column blocks over the columns outside the tiles
     int bcx, bcy, bcz;
     int num_theta;
     int thermal_type;
@@ -69,7 +73,11 @@ struct NuclArgs {
 
 // Launchers (apt_kernels.hip).  All enqueue on `stream` and return the
 // hipError_t of the launch.
-hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream);
+// Main step kernel then the overflow fix-up kernel.  Optional events are
+// recorded before the main kernel, between the two and after the fix-up.
+hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,
+                       hipEvent_t ev_begin = nullptr, hipEvent_t ev_mid = nullptr,
+                       hipEvent_t ev_end = nullptr);
 hipError_t launch_bc_fill(float* ph, int* arg, int fnx, int fny, int fnz, int K, int bcx,
                           int bcy, int bcz, hipStream_t stream);
 hipError_t launch_collect(const float* ph, const int* arg, float* phi, int* alpha, int L, int K,
@@ -85,5 +93,6 @@ hipError_t launch_move_frame(float* ph_old, int* arg_old, float* ph_new, int* ar
 hipError_t launch_nucl_status(const float* phi0, int* status, int cnx, int cny, int cnz, int p,
                               int fnx, int fny, hipStream_t stream);
 hipError_t launch_nucl(const NuclArgs& a, hipStream_t stream);
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
 
 }  // namespace cupf

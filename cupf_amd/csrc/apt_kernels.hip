@@ -296,16 +296,14 @@ __device__ __forceinline__ float slot_update(const StepArgs& a, int gid, float p
 }
 
 // Pass B: right-hand side and explicit Euler update of every slot of one point
-// (3DgrainMPI/APTPhaseField.cu:238-351), loading exactly the phi values each
-// filled slot needs.
-template <int K>
-__device__ __forceinline__ void rhs_point(const StepArgs& a, int i, int j, int k,
-                                          const int (&b)[7], const Gather<K>& g, float a_s,
-                                          float epsv)
+// (3DgrainMPI/APTPhaseField.cu:238-351).  phi_at(S, p) returns the phi of
+// neighbour slot p at stencil row S; only the filled slots' values are read.
+template <int K, class PhiAt>
+__device__ __forceinline__ void rhs_point(const StepArgs& a, int i, int j, int k, int C,
+                                          const Gather<K>& g, float a_s, float epsv, PhiAt&& phi_at)
 {
     float* __restrict__ phn = a.ph_new;
     int* __restrict__ agn = a.arg_new;
-    const int C = b[3];
     const int L = a.L;
     if (g.n == 0) {
 #pragma unroll
@@ -323,7 +321,7 @@ __device__ __forceinline__ void rhs_point(const StepArgs& a, int i, int j, int k
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         const uint32_t p = (g.pm[3] >> (4 * q)) & 15u;
-        pc[q] = (q < g.n && p != 15u) ? a.ph[C + (int)p * L] : -1.0f;
+        pc[q] = (q < g.n && p != 15u) ? phi_at(3, p) : -1.0f;
     }
 #pragma unroll
     for (int q = 0; q < K; ++q) {
@@ -341,7 +339,7 @@ __device__ __forceinline__ void rhs_point(const StepArgs& a, int i, int j, int k
                 v[s] = pc[q];
             } else {
                 const uint32_t p = (g.pm[s] >> (4 * q)) & 15u;
-                v[s] = (p != 15u) ? a.ph[b[s] + (int)p * L] : -1.0f;
+                v[s] = (p != 15u) ? phi_at(s, p) : -1.0f;
             }
         }
         const int gid = g.la[q];
@@ -359,6 +357,72 @@ __device__ __forceinline__ void rhs_point(const StepArgs& a, int i, int j, int k
         agn[o] = ((double)nv < kKill) ? -1 : gid;
         __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+// Pass B into registers: out_id/out_ph hold the new slot values and wr is
+// false only for slots the reference leaves untouched (the thermal
+// `continue`).  Lets the caller issue one straight-line block of stores.
+template <int K, class PhiAt>
+__device__ __forceinline__ void rhs_point_regs(const StepArgs& a, int i, int j, int k,
+                                               const Gather<K>& g, float a_s, float epsv,
+                                               PhiAt&& phi_at, int (&out_id)[K],
+                                               float (&out_ph)[K], bool& wr)
+{
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        out_id[q] = -1;
+        out_ph[q] = -1.0f;
+    }
+    wr = true;
+    if (g.n == 0) return;
+    float T;
+    const bool skip = thermal_T(a, i, j, k, T);
+    wr = !skip;
+    if (skip) return;
+    const float Up = T / a.L_cp;
+    float pc[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const uint32_t p = (g.pm[3] >> (4 * q)) & 15u;
+        pc[q] = (q < g.n && p != 15u) ? phi_at(3, p) : -1.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        if (q >= g.n) continue;
+        float v[7];
+#pragma unroll
+        for (int s = 0; s < 7; ++s) {
+            if (s == 3) {
+                v[s] = pc[q];
+            } else {
+                const uint32_t p = (g.pm[s] >> (4 * q)) & 15u;
+                v[s] = (p != 15u) ? phi_at(s, p) : -1.0f;
+            }
+        }
+        const int gid = g.la[q];
+        float repul = 0.0f;
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+            if (p != q && p < g.n) {
+                const float w = pc[p] + 1.0f;
+                repul += 0.25f * w * w;
+            }
+        }
+        const float nv = slot_update(a, gid, v[0], v[1], v[2], v[3], v[4], v[5], v[6], repul, Up,
+                                     a_s, epsv);
+        out_ph[q] = nv;
+        out_id[q] = ((double)nv < kKill) ? -1 : gid;
+    }
+}
+
+// Direct-path pass B: phi straight from global memory.
+template <int K>
+__device__ __forceinline__ void rhs_point(const StepArgs& a, int i, int j, int k,
+                                          const int (&b)[7], const Gather<K>& g, float a_s,
+                                          float epsv)
+{
+    rhs_point<K>(a, i, j, k, b[3], g, a_s, epsv,
+                 [&](int s, uint32_t p) { return a.ph[b[s] + (int)p * a.L]; });
 }
 
 // Overflow policy for a point whose neighbourhood holds more than K distinct
@@ -595,6 +659,275 @@ __global__ __launch_bounds__(kBlock) void apt_step(StepArgs a)
                 a.fix_list[base + rank] = (unsigned)(i + j * a.fnx + k * A);
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------- tiled ---
+// LDS-tiled z-march.  A workgroup owns a TX x TY tile of interior columns
+// and a chunk of z planes.  The grain ids of the tile plus a one-point halo
+// (boundary maps applied while loading) live in a 3-plane LDS ring; the next
+// plane is prefetched into registers while the current plane is computed, so
+// HBM latency overlaps the gather arithmetic.  phi is read from global memory
+// only for the slots that are filled, which is what APT's sparsity saves.
+constexpr int kTX = 64, kTY = 4, kHX = kTX + 2, kHY = kTY + 2, kNPT = kHX * kHY;
+
+template <int K>
+struct PlaneLoader {
+    static constexpr int NE = K * kNPT;                   // slots per staged plane
+    static constexpr int PER = (NE + kBlock - 1) / kBlock;
+    int off[PER];                                         // global offset minus the plane term
+    int vid[PER];
+    float vph[PER];
+};
+
+template <int K>
+__device__ __forceinline__ void loader_init(PlaneLoader<K>& ld, const StepArgs& a, int i0, int j0)
+{
+#pragma unroll
+    for (int q = 0; q < PlaneLoader<K>::PER; ++q) {
+        const int e = threadIdx.x + q * kBlock;
+        const int s = e / kNPT;
+        const int pt = e - s * kNPT;
+        const int ly = pt / kHX;
+        const int lx = pt - ly * kHX;
+        ld.off[q] = (e < PlaneLoader<K>::NE)
+                        ? bc_map(i0 - 1 + lx, a.fnx, a.bcx) + bc_map(j0 - 1 + ly, a.fny, a.bcy) * a.fnx
+                              + s * a.L
+                        : -1;
+    }
+}
+
+// Elements 0..PER-2 of every lane are always inside the plane; only the last
+// one can fall past NE, and its offset is clamped so the load stays in bounds
+// (the value is then simply not stored).
+template <int K>
+__device__ __forceinline__ void loader_fetch(PlaneLoader<K>& ld, const StepArgs& a, int plane)
+{
+#pragma unroll
+    for (int q = 0; q < PlaneLoader<K>::PER; ++q) {
+        const int o = (q == PlaneLoader<K>::PER - 1 && ld.off[q] < 0) ? 0 : ld.off[q] + plane;
+        ld.vid[q] = a.arg[o];
+        ld.vph[q] = a.ph[o];
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void loader_store(const PlaneLoader<K>& ld, int* ids, float* phs)
+{
+#pragma unroll
+    for (int q = 0; q < PlaneLoader<K>::PER; ++q) {
+        if (q < PlaneLoader<K>::PER - 1 || ld.off[q] >= 0) {
+            ids[threadIdx.x + q * kBlock] = ld.vid[q];
+            phs[threadIdx.x + q * kBlock] = ld.vph[q];
+        }
+    }
+}
+
+// Pass A from LDS for one neighbour point (same probe as scan_point).
+template <int K, int S>
+__device__ __forceinline__ void scan_lds(const int* __restrict__ plane, int pt, Gather<K>& g)
+{
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        const int id = plane[p * kNPT + pt];
+        if (id == -1) continue;
+        int pos = g.n;
+#pragma unroll
+        for (int q = 0; q < K; ++q) pos = (g.la[q] == id) ? q : pos;
+        if (pos >= K) {
+            g.ovf = true;
+            continue;
+        }
+#pragma unroll
+        for (int q = 0; q < K; ++q) g.la[q] = (q == pos) ? id : g.la[q];
+        g.n += (pos == g.n) ? 1 : 0;
+        if (S >= 0) {
+            const uint32_t sh = 4u * (uint32_t)pos;
+            g.pm[S] = (g.pm[S] & ~(15u << sh)) | ((uint32_t)p << sh);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int K, bool G27>
+__device__ __forceinline__ void scan_all_lds(const int* pD, const int* pC, const int* pU, int pt,
+                                             Gather<K>& g)
+{
+    if (!G27) {
+        scan_lds<K, 0>(pD, pt, g);
+        scan_lds<K, 1>(pC, pt - kHX, g);
+        scan_lds<K, 2>(pC, pt - 1, g);
+        scan_lds<K, 3>(pC, pt, g);
+        scan_lds<K, 4>(pC, pt + 1, g);
+        scan_lds<K, 5>(pC, pt + kHX, g);
+        scan_lds<K, 6>(pU, pt, g);
+    } else {
+#define CUPF_L(PL, DX, DY, S) scan_lds<K, S>(PL, pt + (DY) * kHX + (DX), g)
+        CUPF_L(pD, -1, -1, -1); CUPF_L(pD, 0, -1, -1); CUPF_L(pD, 1, -1, -1);
+        CUPF_L(pD, -1, 0, -1);  CUPF_L(pD, 0, 0, 0);   CUPF_L(pD, 1, 0, -1);
+        CUPF_L(pD, -1, 1, -1);  CUPF_L(pD, 0, 1, -1);  CUPF_L(pD, 1, 1, -1);
+        CUPF_L(pC, -1, -1, -1); CUPF_L(pC, 0, -1, 1);  CUPF_L(pC, 1, -1, -1);
+        CUPF_L(pC, -1, 0, 2);   CUPF_L(pC, 0, 0, 3);   CUPF_L(pC, 1, 0, 4);
+        CUPF_L(pC, -1, 1, -1);  CUPF_L(pC, 0, 1, 5);   CUPF_L(pC, 1, 1, -1);
+        CUPF_L(pU, -1, -1, -1); CUPF_L(pU, 0, -1, -1); CUPF_L(pU, 1, -1, -1);
+        CUPF_L(pU, -1, 0, -1);  CUPF_L(pU, 0, 0, 6);   CUPF_L(pU, 1, 0, -1);
+        CUPF_L(pU, -1, 1, -1);  CUPF_L(pU, 0, 1, -1);  CUPF_L(pU, 1, 1, -1);
+#undef CUPF_L
+    }
+}
+
+// Append the overflowed lanes of a wave to the fix-up list (one atomic per wave).
+__device__ __forceinline__ void fixup_append(const StepArgs& a, int ovf, unsigned point)
+{
+    const unsigned long long m = __ballot(ovf);
+    if (m == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(a.fix_count, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (ovf) a.fix_list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = point;
+}
+
+// One z plane of a tile: stage the prefetched plane, gather from LDS, compute
+// every slot into registers, then one straight-line block of 2K stores.  With
+// no vector-memory traffic in the branchy compute part, the wait for the
+// prefetched loads at the top of the next iteration only has to cover those
+// loads, not the stores issued after them.
+template <int K, bool G27>
+__device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld, int (*rid)[K * kNPT],
+                                           float (*rph)[K * kNPT], int i, int j, int k, int k1,
+                                           int pt, float a_s, float epsv)
+{
+    const int A = a.fnx * a.fny;
+    loader_store(ld, rid[(k + 1) % 3], rph[(k + 1) % 3]);
+    __syncthreads();
+    if (k + 2 <= k1) loader_fetch(ld, a, bc_map(k + 2, a.fnz, a.bcz) * A);
+    const int sD = (k + 2) % 3, sC = k % 3, sU = (k + 1) % 3;
+    Gather<K> g;
+    gather_init(g);
+    scan_all_lds<K, G27>(rid[sD], rid[sC], rid[sU], pt, g);
+    const int ovf = g.ovf ? 1 : 0;
+    const int C = i + j * a.fnx + k * A;
+    int oid[K];
+    float oph[K];
+    bool wr = true;
+    if (!ovf) {
+        const float* pD = rph[sD];
+        const float* pC = rph[sC];
+        const float* pU = rph[sU];
+        rhs_point_regs<K>(a, i, j, k, g, a_s, epsv, [&](int s, uint32_t p) {
+            const int o = (int)p * kNPT;
+            switch (s) {
+            case 0: return pD[o + pt];
+            case 1: return pC[o + pt - kHX];
+            case 2: return pC[o + pt - 1];
+            case 3: return pC[o + pt];
+            case 4: return pC[o + pt + 1];
+            case 5: return pC[o + pt + kHX];
+            default: return pU[o + pt];
+            }
+        }, oid, oph, wr);
+    } else {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            oid[q] = -1;
+            oph[q] = -1.0f;
+        }
+    }
+    // Overflowed points are written by the fix-up kernel; skipped slots keep the
+    // destination's old value.  Both send their stores to the per-lane sink.
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const bool w = !ovf && (wr || q >= g.n);
+        int* pa = w ? a.arg_new + C + q * a.L : a.sink_i + lane;
+        float* pp = w ? a.ph_new + C + q * a.L : a.sink_f + lane;
+        *pa = oid[q];
+        *pp = oph[q];
+    }
+    fixup_append(a, ovf, (unsigned)C);
+    __syncthreads();
+}
+
+template <int K, bool G27>
+__device__ __forceinline__ void tile_march(const StepArgs& a, int i0, int j0, int k0, int k1,
+                                           float a_s, float epsv)
+{
+    __shared__ int rid[3][K * kNPT];
+    __shared__ float rph[3][K * kNPT];
+    const int tx = threadIdx.x & (kTX - 1);
+    const int ty = threadIdx.x / kTX;
+    const int i = i0 + tx, j = j0 + ty;
+    const int pt = (ty + 1) * kHX + (tx + 1);
+    const int A = a.fnx * a.fny;
+    PlaneLoader<K> ld;
+    loader_init(ld, a, i0, j0);
+    // prologue: planes k0-1 and k0 into the ring, k0+1 into registers
+    loader_fetch(ld, a, bc_map(k0 - 1, a.fnz, a.bcz) * A);
+    loader_store(ld, rid[(k0 - 1) % 3], rph[(k0 - 1) % 3]);
+    loader_fetch(ld, a, k0 * A);
+    loader_store(ld, rid[k0 % 3], rph[k0 % 3]);
+    loader_fetch(ld, a, bc_map(k0 + 1, a.fnz, a.bcz) * A);
+    // first plane peeled, so the loop is entered in the same state as its back edge
+    tile_plane<K, G27>(a, ld, rid, rph, i, j, k0, k1, pt, a_s, epsv);
+    for (int k = k0 + 1; k < k1; ++k) tile_plane<K, G27>(a, ld, rid, rph, i, j, k, k1, pt, a_s, epsv);
+}
+
+// Columns outside the full tiles (the strip left when fnx-2 or fny-2 is not
+// a multiple of the tile size) take the direct path.
+template <int K, bool G27>
+__device__ __forceinline__ void edge_march(const StepArgs& a, int c, int k0, int k1, float a_s,
+                                           float epsv)
+{
+    const int ex = (a.fnx - 2) - a.tiles_x * kTX;  // leftover columns at the +x end, all rows
+    const int A = a.fnx * a.fny;
+    int i, j;
+    bool active;
+    if (c < ex * (a.fny - 2)) {
+        j = 1 + c / ex;
+        i = 1 + a.tiles_x * kTX + (c - (j - 1) * ex);
+        active = true;
+    } else {
+        const int c2 = c - ex * (a.fny - 2);
+        const int w = a.tiles_x * kTX;  // leftover rows at the +y end, tiled columns only
+        j = 1 + a.tiles_y * kTY + (w > 0 ? c2 / w : 0);
+        i = 1 + (w > 0 ? c2 - (j - 1 - a.tiles_y * kTY) * w : 0);
+        active = w > 0 && j <= a.fny - 2;
+    }
+    if (!active) k1 = k0;
+    for (int k = k0; k < k1; ++k) {
+        const int ovf = apt_point<K, G27>(a, i, j, k, a_s, epsv);
+        fixup_append(a, ovf, (unsigned)(i + j * a.fnx + k * A));
+    }
+}
+
+// APTrhs_psi (3DgrainMPI/APTPhaseField.cu:178-354) over every interior point.
+// Logical workgroups [0, edge) are the edge-column blocks (slowest per column,
+// so dispatched first), the rest are LDS tiles (x fastest, then y, then z chunk).
+template <int K, bool G27>
+__global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
+{
+    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const float a_s = 1.0f + 3.0f * a.kin_delta;
+    const float epsv = -4.0f * a.kin_delta / a_s;
+    const int edge_total = a.edge_blocks * a.nzchunk;
+    if (lb < edge_total) {
+        const int zc = lb / a.edge_blocks;
+        const int cb = lb - zc * a.edge_blocks;
+        const int k0 = 1 + zc * a.kchunk;
+        const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+        edge_march<K, G27>(a, cb * kBlock + threadIdx.x, k0, k1, a_s, epsv);
+    } else {
+        const int ntile = a.tiles_x * a.tiles_y;
+        const int lb2 = lb - edge_total;
+        const int zc = lb2 / ntile;
+        const int r = lb2 - zc * ntile;
+        const int ty = r / a.tiles_x;
+        const int tx = r - ty * a.tiles_x;
+        const int k0 = 1 + zc * a.kchunk;
+        const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+        tile_march<K, G27>(a, 1 + tx * kTX, 1 + ty * kTY, k0, k1, a_s, epsv);
     }
 }
 
@@ -885,18 +1218,32 @@ __global__ __launch_bounds__(kBlock) void nucl_kernel(NuclArgs a)
     }
 }
 
+// Plain streaming copy (16 B per lane per iteration), the reference point for
+// HBM bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(kBlock) void copy_kernel(const float4* __restrict__ src,
+                                                      float4* __restrict__ dst, long long n)
+{
+    for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < n;
+         q += (long long)gridDim.x * kBlock)
+        dst[q] = src[q];
+}
+
 inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
 
-hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream)
+hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,
+                       hipEvent_t ev_begin, hipEvent_t ev_mid, hipEvent_t ev_end)
 {
-    const int nblocks = a.nblk_col * a.nzchunk;
+    const int nblocks = (a.tiles_x * a.tiles_y + a.edge_blocks) * a.nzchunk;
     dim3 grid(nblocks), block(kBlock), fgrid(kFixupBlocks);
 #define CUPF_LAUNCH(KK, GG)                                                                   \
     do {                                                                                      \
-        hipLaunchKernelGGL((apt_step<KK, GG>), grid, block, 0, stream, a);                    \
+        if (ev_begin) (void)hipEventRecord(ev_begin, stream);                                 \
+        hipLaunchKernelGGL((apt_step_tiled<KK, GG>), grid, block, 0, stream, a);              \
+        if (ev_mid) (void)hipEventRecord(ev_mid, stream);                                     \
         hipLaunchKernelGGL((apt_fixup<KK, GG>), fgrid, block, 0, stream, a);                  \
+        if (ev_end) (void)hipEventRecord(ev_end, stream);                                     \
     } while (0)
     if (!gather27) {
         if (K == 5) CUPF_LAUNCH(5, false);
@@ -908,6 +1255,14 @@ hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t strea
         else return hipErrorInvalidValue;
     }
 #undef CUPF_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream)
+{
+    const long long n = (long long)(bytes / 16);
+    hipLaunchKernelGGL(copy_kernel, dim3(256 * 16), dim3(kBlock), 0, stream,
+                       (const float4*)src, (float4*)dst, n);
     return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -36,6 +37,7 @@ struct pf_ctx {
     unsigned long long* counters = nullptr;  // [0] overflow points, [1] nucleation events
     unsigned* fix_list = nullptr;   // overflow fix-up list, one entry per interior point
     unsigned* fix_count = nullptr;  // [0] entries, [1] finished fix-up workgroups
+    int* sink = nullptr;            // 128 words written by lanes that must not store
     long long* sums = nullptr;
     std::vector<long long> h_sums;
     int* alpha_full = nullptr;
@@ -45,6 +47,7 @@ struct pf_ctx {
     float* ball = nullptr;
     int cnx = 0, cny = 0, cnz = 0;
     unsigned long long ncall = 0;
+    int kchunk = 32;                // z planes per workgroup march
     bool have_coords = false, have_orient = false, have_state = false;
 };
 
@@ -104,15 +107,25 @@ StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
     a.overflow = c->counters;
     a.fix_list = c->fix_list;
     a.fix_count = c->fix_count;
+    a.sink_i = c->sink;
+    a.sink_f = reinterpret_cast<float*>(c->sink + 64);
     a.fnx = p.fnx;
     a.fny = p.fny;
     a.fnz = p.fnz;
     a.L = c->L;
     a.ncolx = p.fnx - 2;
     a.ncols = (p.fnx - 2) * (p.fny - 2);
-    a.kchunk = 16;
+    a.kchunk = c->kchunk;
     a.nzchunk = (p.fnz - 2 + a.kchunk - 1) / a.kchunk;
     a.nblk_col = (a.ncols + 255) / 256;
+    a.tiles_x = (p.fnx - 2) / 64;
+    a.tiles_y = (p.fny - 2) / 4;
+    {
+        const int ex = (p.fnx - 2) - a.tiles_x * 64;
+        const int ey = (p.fny - 2) - a.tiles_y * 4;
+        const long long edge_cols = (long long)ex * (p.fny - 2) + (long long)ey * a.tiles_x * 64;
+        a.edge_blocks = (int)((edge_cols + 255) / 256);
+    }
     a.bcx = p.bcx;
     a.bcy = p.bcy;
     a.bcz = p.bcz;
@@ -206,7 +219,7 @@ void free_ctx(pf_ctx* c)
     void* bufs[] = {c->ph[0], c->ph[1], c->arg[0], c->arg[1], c->x, c->y, c->z, c->zbuf,
                     c->R12, c->psi_mac, c->T3d, c->phi0, c->phi, c->alpha, c->counters,
                     c->sums, c->alpha_full, c->loss, c->nstatus, c->ball,
-                    c->fix_list, c->fix_count};
+                    c->fix_list, c->fix_count, c->sink};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -247,6 +260,10 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
 
     pf_ctx* c = new pf_ctx();
     c->p = p;
+    if (const char* kc = getenv("CUPF_KCHUNK")) {
+        const int v = atoi(kc);
+        if (v >= 1 && v <= 4096) c->kchunk = v;
+    }
     c->device = device;
     c->L = (int)L;
     int rc = PF_OK;
@@ -267,7 +284,7 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         (rc = dalloc(&c->phi, (size_t)L)) || (rc = dalloc(&c->alpha, (size_t)L)) ||
         (rc = dalloc(&c->counters, 2)) || (rc = dalloc(&c->sums, p.fnz)) ||
         (rc = dalloc(&c->fix_list, (size_t)(p.fnx - 2) * (p.fny - 2) * (p.fnz - 2))) ||
-        (rc = dalloc(&c->fix_count, 2)))
+        (rc = dalloc(&c->fix_count, 2)) || (rc = dalloc(&c->sink, 128)))
         return bail(rc);
     c->h_sums.resize(p.fnz);
     for (int b = 0; b < 2; ++b) {
@@ -487,6 +504,42 @@ int pf_step_timed(pf_ctx* c, int nsteps, int first_step, int flags, float* ms)
     return PF_OK;
 }
 
+int pf_step_profiled(pf_ctx* c, int nsteps, int first_step, int flags, float* ms_main,
+                     float* ms_fixup)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!c->have_coords || !c->have_orient || !c->have_state)
+        return fail(PF_E_STATE, "pf_step needs coordinates, orientations and an initial state");
+    if (flags & PF_STEP_NUCLEATE) return fail(PF_E_INVALID, "profiled stepping excludes nucleation");
+    const pf_params& p = c->p;
+    std::vector<hipEvent_t> ev(3 * (size_t)nsteps, nullptr);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    const int use_iuc = (flags & PF_STEP_INITIAL_UC) ? 1 : 0;
+    hipError_t err = hipSuccess;
+    for (int q = 0; q < nsteps && err == hipSuccess; ++q) {
+        const int s = first_step + q;
+        const float t = s * p.dt * p.tau0;
+        StepArgs a = make_step_args(c, c->cur, t, use_iuc);
+        err = launch_step(a, p.K, p.gather27, c->stream, ev[3 * q], ev[3 * q + 1], ev[3 * q + 2]);
+        c->cur = 1 - c->cur;
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+    double tm = 0.0, tf = 0.0;
+    for (int q = 0; q < nsteps && err == hipSuccess; ++q) {
+        float a1 = 0.0f, a2 = 0.0f;
+        err = hipEventElapsedTime(&a1, ev[3 * q], ev[3 * q + 1]);
+        if (err == hipSuccess) err = hipEventElapsedTime(&a2, ev[3 * q + 1], ev[3 * q + 2]);
+        tm += a1;
+        tf += a2;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (err != hipSuccess) return fail(PF_E_HIP, std::string("profiled step: ") + hipGetErrorString(err));
+    if (ms_main) *ms_main = (float)tm;
+    if (ms_fixup) *ms_fixup = (float)tf;
+    return PF_OK;
+}
+
 int pf_collect(pf_ctx* c, int which, int flags, float* phi_out, int* alpha_out)
 {
     int rc = ctx_check(c);
@@ -648,6 +701,40 @@ int pf_overflow_count(pf_ctx* c, long long* count)
     HIPCHK(hipStreamSynchronize(c->stream));
     if (count) *count = (long long)v;
     return PF_OK;
+}
+
+int pf_copy_bandwidth(int device, unsigned long long bytes, int iters, float* gbs)
+{
+    if (bytes < 16 || iters < 1 || !gbs) return fail(PF_E_INVALID, "bad arguments");
+    HIPCHK(hipSetDevice(device));
+    bytes &= ~15ull;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PF_OK;
+    float ms = 0.0f;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        rc = fail(PF_E_NOMEM, "copy buffers");
+    } else if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+               hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+               hipMemsetAsync(a, 0, bytes, st) != hipSuccess ||
+               launch_copy(a, b, bytes, st) != hipSuccess ||
+               hipEventRecord(e0, st) != hipSuccess) {
+        rc = fail(PF_E_HIP, "copy set-up failed");
+    } else {
+        for (int q = 0; q < iters && rc == PF_OK; ++q)
+            if (launch_copy(a, b, bytes, st) != hipSuccess) rc = fail(PF_E_HIP, "copy launch failed");
+        if (rc == PF_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = fail(PF_E_HIP, "copy timing failed");
+    }
+    if (rc == PF_OK) *gbs = (float)(2.0 * bytes * iters / (ms * 1e-3) / 1e9);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 int pf_sync(pf_ctx* c)

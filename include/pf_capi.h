@@ -128,6 +128,13 @@ int pf_step(pf_ctx* ctx, int nsteps, int first_step, int flags);
 /* pf_step bracketed by HIP events on the context's stream; *ms = elapsed. */
 int pf_step_timed(pf_ctx* ctx, int nsteps, int first_step, int flags, float* ms);
 
+/* Diagnostic: pf_step with a HIP event pair around every launch of the main
+ * step kernel and of the overflow fix-up kernel; the summed elapsed times
+ * are returned in ms.  No nucleation.  No counterpart in the reference (its
+ * only timer is the wall clock around the loop, 3DgrainMPI/APTPhaseField.cu:850,940). */
+int pf_step_profiled(pf_ctx* ctx, int nsteps, int first_step, int flags, float* ms_main,
+                     float* ms_fixup);
+
 /* pf_collect flags */
 #define PF_COLLECT_CLEAR_ALPHA 1  /* zero alpha first (the cudaMemset at
                                      3DgrainMPI/APTPhaseField.cu:663,809,820,907);
@@ -168,6 +175,11 @@ int pf_overflow_count(pf_ctx* ctx, long long* count);
 
 /* Wait for all work on the context's stream. */
 int pf_sync(pf_ctx* ctx);
+
+/* Diagnostic: bandwidth of a plain device-to-device copy kernel over `bytes`
+ * (read + write counted), iters launches; the reference point for the HBM
+ * roofline and for scaling HBM byte counters.  No counterpart in the reference. */
+int pf_copy_bandwidth(int device, unsigned long long bytes, int iters, float* gbs);
 
 /* Library build information, e.g. "cupf_amd gfx950 K={5,8}". */
 const char* pf_build_info(void);

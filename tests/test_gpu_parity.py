@@ -45,6 +45,22 @@ def test_step_small_bitexact(K):
     assert np.array_equal(gi, oi)
 
 
+@pytest.mark.parametrize("shape,K,g27,bc", [
+    ((127, 7, 10), 8, 0, (1, 1, 0)),    # 2x2 full 64x4 tiles, no edge columns
+    ((130, 9, 12), 5, 0, (0, 1, 0)),    # tiles plus leftover columns in x and rows in y
+    ((63, 12, 40), 8, 0, (1, 0, 1)),    # one tile column, leftover row, two z chunks
+    ((130, 9, 12), 8, 1, (1, 1, 0)),    # 27-point gather through the LDS tiles
+])
+def test_tiled_path_shapes(shape, K, g27, bc):
+    c = cases.make_line_case("tiles", n=shape, grains=12, K=K, seed=43, z0_cells=shape[2] // 2,
+                             bc=bc, gather27=g27,
+                             thermal_type=capi.THERMAL_LINE_3DGRAIN if g27 else capi.THERMAL_LINE)
+    ctx, sim = _run_pair(c, 12)
+    bad, dphi = compare_states(ctx, sim)
+    assert bad == 0, first_mismatch(ctx, sim, K, c.params["fnx"], c.params["fny"], c.params["fnz"])
+    assert dphi == 0.0
+
+
 @pytest.mark.parametrize("bc", [(1, 1, 0), (0, 0, 0), (1, 0, 1)])
 def test_step_boundary_conditions(bc):
     c = cases.make_line_case("bc", n=(12, 14, 10), grains=4, K=5, seed=7, z0_cells=5, bc=bc)
