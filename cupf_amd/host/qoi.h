@@ -1,0 +1,41 @@
+// qoi.h -- host quantities of interest computed from the grain-id field alpha
+// (3DgrainMPI/QOI.h:12-48): per-grain volume, quadruple-junction search,
+// line-mode tip height / areas / cross-sections, moving-window tip samples.
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "params.h"
+
+class QOI {
+public:
+    virtual ~QOI() = default;
+    virtual void searchJunctionsOnImage(const GlobalConstants& params, const int* alpha) = 0;
+    void calculateLineQoIs(const GlobalConstants& params, int& cur_tip, const int* alpha, int kt,
+                           const float* z, const int* loss_area, int move_count);
+    void calculateQoIs(const GlobalConstants& params, const int* alpha, int kt);
+    void sampleHeights(int& cur_tip, const int* alpha, int fnx, int fny, int fnz);
+    std::map<std::string, std::vector<int>> mQoIVectorIntData;
+    std::map<std::string, std::vector<float>> mQoIVectorFloatData;
+    int mNumActiveGrains = 0;
+};
+
+class QOILine : public QOI {
+public:
+    explicit QOILine(const GlobalConstants& params);
+    void searchJunctionsOnImage(const GlobalConstants& params, const int* alpha) override;
+
+private:
+    int mNumNodeFeatures = 8;
+};
+
+class QOI3D : public QOI {
+public:
+    explicit QOI3D(const GlobalConstants& params);
+    void searchJunctionsOnImage(const GlobalConstants& params, const int* alpha) override;
+
+private:
+    int mNumNodeFeatures = 9;
+};

@@ -1,0 +1,34 @@
+"""QoI restatement (numpy) -- TEST INFRASTRUCTURE ONLY.
+
+Follows 3DgrainMPI/QOI.cpp: per-grain volume over interior points with
+alpha > 0 in first-seen (k, j, i) scan order (:245-278); 3-D junctions: solid
+interior points whose 19-point neighbourhood holds no liquid and exactly 4
+distinct ids with the largest count <= 10 (:280-358); ids compared as sorted
+sets (the reference's order comes from unordered_map iteration).
+"""
+import numpy as np
+
+
+def volume(alpha, fnx, fny, fnz):
+    a = np.asarray(alpha).reshape(fnz, fny, fnx)[1:-1, 1:-1, 1:-1].ravel()
+    a = a[a > 0]
+    ids, first = np.unique(a, return_index=True)
+    order = np.argsort(first)
+    counts = np.bincount(np.searchsorted(ids, a), minlength=ids.size)
+    return ids[order].tolist(), counts[order].tolist()
+
+
+def junctions3d(alpha, fnx, fny, fnz):
+    A = np.asarray(alpha).reshape(fnz, fny, fnx)
+    offs = [(dk, dj, di) for dk in (-1, 0, 1) for dj in (-1, 0, 1) for di in (-1, 0, 1)
+            if di * di + dj * dj + dk * dk <= 2]
+    core = A[1:-1, 1:-1, 1:-1]
+    nb = np.stack([A[1 + dk:fnz - 1 + dk, 1 + dj:fny - 1 + dj, 1 + di:fnx - 1 + di] for dk, dj, di in offs])
+    out = []
+    cand = np.argwhere((core != 0) & ~(nb == 0).any(0))
+    for k, j, i in cand:
+        v = nb[:, k, j, i]
+        ids, cnt = np.unique(v, return_counts=True)
+        if ids.size == 4 and cnt.max() <= 10:
+            out.append((int(i + 1), int(j + 1), int(k + 1), int(cnt.max()), tuple(int(x) for x in ids)))
+    return out
