@@ -6,10 +6,16 @@ field G=1e6 K/m, R=0.1 m/s, BC 110), state resident in HBM.  A step is one
 APTrhs_psi time step over all (nx+1)(ny+1)(nz+1) updated points.
 
 Prints one JSON line:
-  value      Mcell-steps/s over the timed steps (whole job, one GPU)
-  roofline   the main step kernel's algorithmic HBM bytes (updated points x
-             K x 16 B: one read and one write of every slot's float32 phi and
-             int32 id) over its average HIP-event duration, against 8 TB/s
+  value      Mcell-steps/s over the timed steps (whole job, one GPU; every
+             updated point of the domain counts, as every one is updated)
+  roofline   the main step kernel's algorithmic HBM bytes over its average
+             HIP-event duration, against 8 TB/s.  The kernel skips z planes of
+             64x4 column tiles whose stencil reaches only empty slots and whose
+             destination is empty (their result is known without reading
+             anything), so its algorithmic bytes are those of the points it
+             processes: processed points x K x 16 B (one read and one write of
+             every slot's float32 phi and int32 id).  The same figure for all
+             updated points (the dense count) is given as dense_equiv_gbs.
   cpu_baseline  the CPU oracle (float32 restatement, OpenMP) on the same
              post-warmup state for a few steps
 """
@@ -42,6 +48,47 @@ def traffic_from_profiles(workload: str):
             return json.load(f).get(workload, {}).get("hbm_bytes_per_launch")
     except Exception:
         return None
+
+
+def _bc_map(i: int, n: int, bc: int) -> int:
+    if i == 0:
+        return n - 2 if bc else 2
+    if i == n - 1:
+        return 1 if bc else n - 3
+    return i
+
+
+def processed_points(ctx, p) -> int:
+    """Points in the tile planes the next step processes: those whose 3x3
+    tile neighbourhood at k-1, k, k+1 (through the boundary maps) is active in
+    the source buffer, or whose own tile plane is active in the destination."""
+    fnx, fny, fnz = int(p["fnx"]), int(p["fny"]), int(p["fnz"])
+    bcx, bcy, bcz = int(p["bcx"]), int(p["bcy"]), int(p["bcz"])
+    src = ctx.activity(0, with_map=True)[2].astype(bool)
+    dst = ctx.activity(1, with_map=True)[2].astype(bool)
+    nz, nty, ntx = src.shape
+    tx_of = lambda i: (i - 1) // 64
+    ty_of = lambda j: (j - 1) // 4
+    # tile-neighbour index lists through the boundary maps
+    xs = [(tx_of(_bc_map(1 + 64 * t - 1, fnx, bcx)), t, tx_of(_bc_map(min(1 + 64 * t + 64, fnx - 1), fnx, bcx)))
+          for t in range(ntx)]
+    ys = [(ty_of(_bc_map(1 + 4 * t - 1, fny, bcy)), t, ty_of(_bc_map(min(1 + 4 * t + 4, fny - 1), fny, bcy)))
+          for t in range(nty)]
+    dil = np.zeros_like(src)
+    for ty in range(nty):
+        for tx in range(ntx):
+            acc = np.zeros(nz, bool)
+            for yy in ys[ty]:
+                for xx in xs[tx]:
+                    acc |= src[:, yy, xx]
+            dil[:, ty, tx] = acc
+    proc = np.zeros_like(src)
+    for k in range(1, fnz - 1):
+        proc[k] = dil[_bc_map(k - 1, fnz, bcz)] | dil[k] | dil[_bc_map(k + 1, fnz, bcz)] | dst[k]
+    wx = np.array([min(64, fnx - 2 - 64 * t) for t in range(ntx)])
+    wy = np.array([min(4, fny - 2 - 4 * t) for t in range(nty)])
+    pts = np.outer(wy, wx)
+    return int((proc[1:fnz - 1] * pts[None]).sum())
 
 
 def main() -> int:
@@ -101,15 +148,23 @@ def main() -> int:
 
     # per-kernel durations (HIP events around every launch on the context's stream)
     nprof = min(args.steps, 100)
+    proc0 = processed_points(ctx, p)
     ms_main, ms_fix = ctx.step_profiled(nprof, step0)
     step0 += nprof
+    proc1 = processed_points(ctx, p)
+    proc = (proc0 + proc1) // 2
     avg_main = ms_main / nprof
-    log(f"[bench] per launch: apt_step {avg_main:.4f} ms, apt_fixup {ms_fix / nprof:.4f} ms")
+    act, tot = ctx.activity(0)
+    log(f"[bench] per launch: apt_step {avg_main:.4f} ms, apt_fixup {ms_fix / nprof:.4f} ms; "
+        f"processed points {proc0}..{proc1} of {n_upd} ({proc / n_upd:.1%}), "
+        f"active tile planes {act}/{tot}")
 
     copy_gbs = capi.copy_bandwidth(1 << 30, 10)
     log(f"[bench] plain copy kernel: {copy_gbs:.0f} GB/s")
-    bytes_step = n_upd * args.K * 16
+    bytes_step = proc * args.K * 16
+    bytes_dense = n_upd * args.K * 16
     achieved = bytes_step / (avg_main * 1e-3) / 1e9
+    dense_equiv = bytes_dense / (avg_main * 1e-3) / 1e9
     mcell = n_upd * args.steps / wall / 1e6
 
     cpu = None
@@ -170,6 +225,10 @@ def main() -> int:
             "frac_of_copy": round(achieved / copy_gbs, 4),
             "kernel_ms": round(avg_main, 4),
             "algorithmic_bytes_per_launch": bytes_step,
+            "processed_points_per_launch": proc,
+            "updated_points_per_launch": n_upd,
+            "dense_algorithmic_bytes_per_launch": bytes_dense,
+            "dense_equiv_gbs": round(dense_equiv, 1),
         },
         "cpu_baseline": cpu,
     }

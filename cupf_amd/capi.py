@@ -110,6 +110,7 @@ def load() -> ctypes.CDLL:
         "pf_nucl_events": ([vp, llp], c_int),
         "pf_overflow_count": ([vp, llp], c_int),
         "pf_sync": ([vp], c_int),
+        "pf_activity": ([vp, c_int, ctypes.c_void_p, llp, llp], c_int),
         "pf_copy_bandwidth": ([c_int, ctypes.c_ulonglong, c_int, fp], c_int),
     }
     for name, (argt, rest) in sig.items():
@@ -276,3 +277,14 @@ class APTContext:
         v = ctypes.c_longlong()
         _check(load().pf_overflow_count(self._h, ctypes.byref(v)))
         return v.value
+
+    def activity(self, which: int = 0, with_map: bool = False):
+        """(active tile planes, all interior tile planes[, map]) of a buffer's
+        activity map; map is uint8 [fnz, nty, ntx] (64x4 column tiles)."""
+        a, t = ctypes.c_longlong(), ctypes.c_longlong()
+        p = self.params
+        ntx, nty = (int(p["fnx"]) - 2 + 63) // 64, (int(p["fny"]) - 2 + 3) // 4
+        m = np.zeros((int(p["fnz"]), nty, ntx), np.uint8) if with_map else None
+        _check(load().pf_activity(self._h, which, None if m is None else m.ctypes.data,
+                                  ctypes.byref(a), ctypes.byref(t)))
+        return (a.value, t.value, m) if with_map else (a.value, t.value)

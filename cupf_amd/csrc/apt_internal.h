@@ -30,11 +30,21 @@ struct StepArgs {
     unsigned* fix_count;   // [0] entries in fix_list, [1] fix-up workgroups finished
     int* sink_i;           // 64-lane write sinks for lanes that must not store
     float* sink_f;
+    // Activity map: one word per (column tile, z plane) and buffer.  A tile
+    // plane of a buffer may hold a slot other than the empty pair (-1, -1.0f)
+    // only if its word is >= that buffer's epoch.  Epochs come from one
+    // increasing counter, so stale words never need clearing.
+    const unsigned* act_src;
+    unsigned* act_dst;
+    unsigned ep_src;       // epoch of the source buffer
+    unsigned ep_dst;       // epoch the destination buffer had before this step
+    unsigned ep_new;       // epoch the destination buffer gets from this step
+    int use_act;           // 0: process every plane (the map is still maintained)
+    int ntx, nty, ntiles;  // activity tiles (64x4, partial ones included)
     int fnx, fny, fnz;
     int L;                 // slot stride fnx*fny*fnz
     int ncolx, ncols;      // interior columns: (fnx-2), (fnx-2)*(fny-2)
     int nzchunk, kchunk;   // z-chunks per column, planes per chunk
-    int nblk_col;          // column blocks (direct path)
     int tiles_x, tiles_y;  // full 64x4 column tiles (LDS path)
     int edge_blocks;       // 256-column blocks over the columns outside the tiles
     int bcx, bcy, bcz;
@@ -60,6 +70,9 @@ struct NuclArgs {
     const float* T3d;
     const float* ball;     // (2p+1)^3 tanh template
     unsigned long long* events;
+    unsigned* act;         // activity map of the buffer nucleated into, and its epoch
+    unsigned ep;
+    int ntx, ntiles;
     int fnx, fny, fnz, L, K;
     int cnx, cny, cnz, p;
     int use_iuc;
@@ -93,6 +106,10 @@ hipError_t launch_move_frame(float* ph_old, int* arg_old, float* ph_new, int* ar
 hipError_t launch_nucl_status(const float* phi0, int* status, int cnx, int cny, int cnz, int p,
                               int fnx, int fny, hipStream_t stream);
 hipError_t launch_nucl(const NuclArgs& a, hipStream_t stream);
+// Activity map of one buffer from its contents: every tile plane holding a
+// non-empty slot gets `ep`.
+hipError_t launch_act_rebuild(const float* ph, const int* arg, unsigned* act, unsigned ep, int fnx,
+                              int fny, int fnz, int K, int ntx, int ntiles, hipStream_t stream);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
 
 }  // namespace cupf

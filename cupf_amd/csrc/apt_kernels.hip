@@ -584,9 +584,10 @@ __device__ __forceinline__ void gather_overflow(const StepArgs& a, const Nbr& nb
     }
 }
 
-// One interior point: pass A, then pass B.  A point whose neighbourhood
-// overflows K slots is left untouched here and returned as 1; the fix-up
-// kernel redoes it under the overflow policy.
+// One interior point: pass A, then pass B.  Returns 0 if no neighbour holds a
+// grain id (every slot was written as the empty pair), 1 if slots were
+// computed, 2 if the neighbourhood overflows K slots: the point is then left
+// untouched here and the fix-up kernel redoes it under the overflow policy.
 template <int K, bool G27>
 __device__ __forceinline__ int apt_point(const StepArgs& a, int i, int j, int k, float a_s,
                                          float epsv)
@@ -595,11 +596,11 @@ __device__ __forceinline__ int apt_point(const StepArgs& a, int i, int j, int k,
     Gather<K> g;
     gather_init(g);
     scan_all<K, G27>(a, nb, g);
-    if (g.ovf) return 1;
+    if (g.ovf) return 2;
     int b[7];
     stencil_bases(a, nb, b);
     rhs_point<K>(a, i, j, k, b, g, a_s, epsv);
-    return 0;
+    return g.n > 0 ? 1 : 0;
 }
 
 // The same point under the overflow policy (fix-up kernel only).
@@ -615,61 +616,25 @@ __device__ __forceinline__ void apt_point_overflow(const StepArgs& a, int i, int
     rhs_point<K>(a, i, j, k, b, g, a_s, epsv);
 }
 
-// Workgroup id -> logical block, so that consecutive logical blocks land on
-// the same XCD (hardware dispatches workgroup b to XCD b % 8): the B/T
-// neighbour rows shared by adjacent column blocks then stay in one L2.
-__device__ __forceinline__ int xcd_logical_block(int b, int nb)
-{
-    const int q = nb >> 3, r = nb & 7, x = b & 7;
-    return x * q + min(x, r) + (b >> 3);
-}
-
-// APTrhs_psi (3DgrainMPI/APTPhaseField.cu:178-354).  One lane owns one
-// interior (i,j) column and marches a chunk of z planes, so the planes it
-// reads as U are re-read as C and D from L1/L2 on the next iterations.  The
-// halo is never read: neighbour indices go through the per-axis boundary
-// maps, which replaces the separate APTsetBC3D pass.
-template <int K, bool G27>
-__global__ __launch_bounds__(kBlock) void apt_step(StepArgs a)
-{
-    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int zc = lb / a.nblk_col;
-    const int cb = lb - zc * a.nblk_col;
-    const int c = cb * kBlock + threadIdx.x;
-    const bool active = c < a.ncols;
-    const int j = 1 + c / a.ncolx;
-    const int i = 1 + (c - (j - 1) * a.ncolx);
-    const int k0 = 1 + zc * a.kchunk;
-    const int k1 = active ? min(k0 + a.kchunk, a.fnz - 1) : k0;
-    const float a_s = 1.0f + 3.0f * a.kin_delta;
-    const float epsv = -4.0f * a.kin_delta / a_s;
-    const int A = a.fnx * a.fny;
-    for (int k = k0; k < k1; ++k) {
-        const int ovf = apt_point<K, G27>(a, i, j, k, a_s, epsv);
-        // Append overflowed points to the fix-up list: one atomic per wave.
-        const unsigned long long m = __ballot(ovf);
-        if (m != 0ull) {
-            const int lane = threadIdx.x & 63;
-            const int leader = __ffsll((long long)m) - 1;
-            unsigned base = 0;
-            if (lane == leader) base = atomicAdd(a.fix_count, (unsigned)__popcll(m));
-            base = __shfl(base, leader);
-            if (ovf) {
-                const unsigned rank = (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-                a.fix_list[base + rank] = (unsigned)(i + j * a.fnx + k * A);
-            }
-        }
-    }
-}
-
 // ---------------------------------------------------------------- tiled ---
 // LDS-tiled z-march.  A workgroup owns a TX x TY tile of interior columns
-// and a chunk of z planes.  The grain ids of the tile plus a one-point halo
+// and a chunk of z planes.  The ids and phi of the tile plus a one-point halo
 // (boundary maps applied while loading) live in a 3-plane LDS ring; the next
 // plane is prefetched into registers while the current plane is computed, so
-// HBM latency overlaps the gather arithmetic.  phi is read from global memory
-// only for the slots that are filled, which is what APT's sparsity saves.
+// HBM latency overlaps the gather arithmetic.
+//
+// Planes are skipped with the activity map.  If every tile plane that the
+// stencil of plane k can reach (the 3 x 3 neighbouring tiles at k-1, k, k+1,
+// through the boundary maps) holds only empty pairs in the source, every
+// point of plane k gathers no id and its result is K empty pairs; if the
+// destination tile plane is also all empty, writing them changes nothing.
+// Such planes are neither loaded nor written, and a source plane known to be
+// all empty is staged as empty pairs without reading memory.  The results are
+// the same bits as processing everything.
 constexpr int kTX = 64, kTY = 4, kHX = kTX + 2, kHY = kTY + 2, kNPT = kHX * kHY;
+
+__device__ __forceinline__ int act_tile_x(int i) { return (i - 1) / kTX; }
+__device__ __forceinline__ int act_tile_y(int j) { return (j - 1) / kTY; }
 
 template <int K>
 struct PlaneLoader {
@@ -699,10 +664,20 @@ __device__ __forceinline__ void loader_init(PlaneLoader<K>& ld, const StepArgs& 
 
 // Elements 0..PER-2 of every lane are always inside the plane; only the last
 // one can fall past NE, and its offset is clamped so the load stays in bounds
-// (the value is then simply not stored).
+// (the value is then simply not stored).  An inactive plane (all empty pairs
+// in the source) is staged without reading memory.
 template <int K>
-__device__ __forceinline__ void loader_fetch(PlaneLoader<K>& ld, const StepArgs& a, int plane)
+__device__ __forceinline__ void loader_fetch(PlaneLoader<K>& ld, const StepArgs& a, int plane,
+                                             bool active)
 {
+    if (!active) {
+#pragma unroll
+        for (int q = 0; q < PlaneLoader<K>::PER; ++q) {
+            ld.vid[q] = -1;
+            ld.vph[q] = -1.0f;
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < PlaneLoader<K>::PER; ++q) {
         const int o = (q == PlaneLoader<K>::PER - 1 && ld.off[q] < 0) ? 0 : ld.off[q] + plane;
@@ -793,16 +768,19 @@ __device__ __forceinline__ void fixup_append(const StepArgs& a, int ovf, unsigne
 // every slot into registers, then one straight-line block of 2K stores.  With
 // no vector-memory traffic in the branchy compute part, the wait for the
 // prefetched loads at the top of the next iteration only has to cover those
-// loads, not the stores issued after them.
+// loads, not the stores issued after them.  `pf_next` says whether plane k+2
+// is fetched (only when plane k+1 is processed too); `act_next2` whether that
+// plane holds anything in the source.
 template <int K, bool G27>
 __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld, int (*rid)[K * kNPT],
-                                           float (*rph)[K * kNPT], int i, int j, int k, int k1,
-                                           int pt, float a_s, float epsv)
+                                           float (*rph)[K * kNPT], int i, int j, int k, int tile,
+                                           int pt, float a_s, float epsv, bool pf_next,
+                                           bool act_next2)
 {
     const int A = a.fnx * a.fny;
     loader_store(ld, rid[(k + 1) % 3], rph[(k + 1) % 3]);
     __syncthreads();
-    if (k + 2 <= k1) loader_fetch(ld, a, bc_map(k + 2, a.fnz, a.bcz) * A);
+    if (pf_next) loader_fetch(ld, a, bc_map(k + 2, a.fnz, a.bcz) * A, act_next2);
     const int sD = (k + 2) % 3, sC = k % 3, sU = (k + 1) % 3;
     Gather<K> g;
     gather_init(g);
@@ -846,36 +824,96 @@ __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld
         *pa = oid[q];
         *pp = oph[q];
     }
+    // A point that gathered an id may leave non-empty slots (an overflowed one
+    // will, through the fix-up): mark the destination tile plane.
+    if (ovf || g.n > 0) a.act_dst[k * a.ntiles + tile] = a.ep_new;
     fixup_append(a, ovf, (unsigned)C);
     __syncthreads();
 }
 
+// Planes of the chunk [k0, k1) to process, bit m for plane k0+m, and the
+// source activity of planes k0-1 .. k1, bit r for plane k0-1+r (through the
+// z boundary map).  Source words of the 3 x 3 tiles around (tx, ty) at the
+// mapped planes, destination words of the own tile.  Every thread of the
+// workgroup gets the same masks.
+__device__ __forceinline__ void tile_need(const StepArgs& a, int tx, int ty, int i0, int j0, int k0,
+                                          int k1, uint64_t& need, uint64_t& act)
+{
+    __shared__ unsigned long long s_act, s_dst;
+    const int n = k1 - k0;
+    const uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+    if (!a.use_act) {
+        need = full;
+        act = ~0ull;
+        return;
+    }
+    if (threadIdx.x == 0) {
+        s_act = 0ull;
+        s_dst = 0ull;
+    }
+    __syncthreads();
+    const int xl = act_tile_x(bc_map(i0 - 1, a.fnx, a.bcx)), xr = act_tile_x(bc_map(i0 + kTX, a.fnx, a.bcx));
+    const int yl = act_tile_y(bc_map(j0 - 1, a.fny, a.bcy)), yr = act_tile_y(bc_map(j0 + kTY, a.fny, a.bcy));
+    for (int t = threadIdx.x; t < 9 * (n + 2); t += kBlock) {
+        const int r = t / 9, q = t - 9 * r;
+        const int qx = q % 3, qy = q / 3;
+        const int ttx = qx == 0 ? xl : (qx == 1 ? tx : xr);
+        const int tty = qy == 0 ? yl : (qy == 1 ? ty : yr);
+        const int kk = bc_map(k0 - 1 + r, a.fnz, a.bcz);
+        if (a.act_src[kk * a.ntiles + ttx + tty * a.ntx] >= a.ep_src) atomicOr(&s_act, 1ull << r);
+    }
+    const int own = tx + ty * a.ntx;
+    if ((int)threadIdx.x < n && a.act_dst[(k0 + threadIdx.x) * a.ntiles + own] >= a.ep_dst)
+        atomicOr(&s_dst, 1ull << threadIdx.x);
+    __syncthreads();
+    act = s_act;
+    need = ((act | (act >> 1) | (act >> 2)) | s_dst) & full;
+}
+
 template <int K, bool G27>
-__device__ __forceinline__ void tile_march(const StepArgs& a, int i0, int j0, int k0, int k1,
+__device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, int k0, int k1,
                                            float a_s, float epsv)
 {
     __shared__ int rid[3][K * kNPT];
     __shared__ float rph[3][K * kNPT];
-    const int tx = threadIdx.x & (kTX - 1);
-    const int ty = threadIdx.x / kTX;
-    const int i = i0 + tx, j = j0 + ty;
-    const int pt = (ty + 1) * kHX + (tx + 1);
+    const int i0 = 1 + tx * kTX, j0 = 1 + ty * kTY;
+    uint64_t need, act;
+    tile_need(a, tx, ty, i0, j0, k0, k1, need, act);
+    if (need == 0ull) return;
+    const int lx = threadIdx.x & (kTX - 1);
+    const int ly = threadIdx.x / kTX;
+    const int i = i0 + lx, j = j0 + ly;
+    const int pt = (ly + 1) * kHX + (lx + 1);
     const int A = a.fnx * a.fny;
+    const int tile = tx + ty * a.ntx;
+    auto act_at = [&](int kk) { return kk <= k1 && ((act >> (kk - k0 + 1)) & 1ull) != 0ull; };
+    auto need_at = [&](int kk) { return kk < k1 && ((need >> (kk - k0)) & 1ull) != 0ull; };
     PlaneLoader<K> ld;
     loader_init(ld, a, i0, j0);
-    // prologue: planes k0-1 and k0 into the ring, k0+1 into registers
-    loader_fetch(ld, a, bc_map(k0 - 1, a.fnz, a.bcz) * A);
-    loader_store(ld, rid[(k0 - 1) % 3], rph[(k0 - 1) % 3]);
-    loader_fetch(ld, a, k0 * A);
-    loader_store(ld, rid[k0 % 3], rph[k0 % 3]);
-    loader_fetch(ld, a, bc_map(k0 + 1, a.fnz, a.bcz) * A);
-    // first plane peeled, so the loop is entered in the same state as its back edge
-    tile_plane<K, G27>(a, ld, rid, rph, i, j, k0, k1, pt, a_s, epsv);
-    for (int k = k0 + 1; k < k1; ++k) tile_plane<K, G27>(a, ld, rid, rph, i, j, k, k1, pt, a_s, epsv);
+    int k = k0;
+    while (k < k1) {
+        if (!need_at(k)) {
+            ++k;
+            continue;
+        }
+        // start of a run of processed planes: k-1 and k into the ring, k+1 into
+        // registers (the ring slots are free: the previous run ended with a barrier)
+        loader_fetch(ld, a, bc_map(k - 1, a.fnz, a.bcz) * A, act_at(k - 1));
+        loader_store(ld, rid[(k + 2) % 3], rph[(k + 2) % 3]);
+        loader_fetch(ld, a, k * A, act_at(k));
+        loader_store(ld, rid[k % 3], rph[k % 3]);
+        loader_fetch(ld, a, bc_map(k + 1, a.fnz, a.bcz) * A, act_at(k + 1));
+        do {
+            tile_plane<K, G27>(a, ld, rid, rph, i, j, k, tile, pt, a_s, epsv, need_at(k + 1),
+                               act_at(k + 2));
+            ++k;
+        } while (need_at(k));
+    }
 }
 
 // Columns outside the full tiles (the strip left when fnx-2 or fny-2 is not
-// a multiple of the tile size) take the direct path.
+// a multiple of the tile size) take the direct path, with the same plane
+// skipping per column.
 template <int K, bool G27>
 __device__ __forceinline__ void edge_march(const StepArgs& a, int c, int k0, int k1, float a_s,
                                            float epsv)
@@ -895,39 +933,69 @@ __device__ __forceinline__ void edge_march(const StepArgs& a, int c, int k0, int
         i = 1 + (w > 0 ? c2 - (j - 1 - a.tiles_y * kTY) * w : 0);
         active = w > 0 && j <= a.fny - 2;
     }
-    if (!active) k1 = k0;
+    if (!active) return;
+    const int n = k1 - k0;
+    const uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+    const int tile = act_tile_x(i) + act_tile_y(j) * a.ntx;
+    uint64_t need = full;
+    if (a.use_act) {
+        const int xl = act_tile_x(bc_map(i - 1, a.fnx, a.bcx)), xc = act_tile_x(i);
+        const int xr = act_tile_x(bc_map(i + 1, a.fnx, a.bcx));
+        const int yl = act_tile_y(bc_map(j - 1, a.fny, a.bcy)), yc = act_tile_y(j);
+        const int yr = act_tile_y(bc_map(j + 1, a.fny, a.bcy));
+        uint64_t act = 0ull, dst = 0ull;
+        for (int r = 0; r < n + 2; ++r) {
+            const unsigned* w = a.act_src + bc_map(k0 - 1 + r, a.fnz, a.bcz) * a.ntiles;
+            bool on = false;
+#pragma unroll
+            for (int qy = 0; qy < 3; ++qy) {
+                const int ty = (qy == 0 ? yl : (qy == 1 ? yc : yr)) * a.ntx;
+                on = on || w[xl + ty] >= a.ep_src || w[xc + ty] >= a.ep_src ||
+                     w[xr + ty] >= a.ep_src;
+            }
+            if (on) act |= 1ull << r;
+            if (r < n && a.act_dst[(k0 + r) * a.ntiles + tile] >= a.ep_dst) dst |= 1ull << r;
+        }
+        need = ((act | (act >> 1) | (act >> 2)) | dst) & full;
+    }
     for (int k = k0; k < k1; ++k) {
-        const int ovf = apt_point<K, G27>(a, i, j, k, a_s, epsv);
-        fixup_append(a, ovf, (unsigned)(i + j * a.fnx + k * A));
+        if (!((need >> (k - k0)) & 1ull)) continue;
+        const int r = apt_point<K, G27>(a, i, j, k, a_s, epsv);
+        if (r != 0) a.act_dst[k * a.ntiles + tile] = a.ep_new;
+        fixup_append(a, r == 2, (unsigned)(i + j * a.fnx + k * A));
     }
 }
 
 // APTrhs_psi (3DgrainMPI/APTPhaseField.cu:178-354) over every interior point.
-// Logical workgroups [0, edge) are the edge-column blocks (slowest per column,
-// so dispatched first), the rest are LDS tiles (x fastest, then y, then z chunk).
+// The hardware hands workgroup b to XCD b % 8.  Each XCD gets, for every z
+// chunk in turn, an equal share of that chunk's work: first its edge-column
+// blocks (edge block e goes to XCD e % 8), then a contiguous run of tiles
+// (x fastest), so y-neighbour tiles share halo rows in one L2.  With plane
+// skipping the active planes sit in a few z chunks, and this spreads them
+// over the whole chip instead of the few XCDs a plain linear order would use.
 template <int K, bool G27>
 __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
 {
-    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int x = blockIdx.x & 7, s = blockIdx.x >> 3;
+    const int ebx = (a.edge_blocks + 7) >> 3;             // edge slots per XCD per chunk
+    const int ntile = a.tiles_x * a.tiles_y;
+    const int tpx = (ntile + 7) >> 3;                     // tile slots per XCD per chunk
+    const int per = ebx + tpx;
+    const int zc = s / per, r = s - zc * per;
+    if (zc >= a.nzchunk) return;
     const float a_s = 1.0f + 3.0f * a.kin_delta;
     const float epsv = -4.0f * a.kin_delta / a_s;
-    const int edge_total = a.edge_blocks * a.nzchunk;
-    if (lb < edge_total) {
-        const int zc = lb / a.edge_blocks;
-        const int cb = lb - zc * a.edge_blocks;
-        const int k0 = 1 + zc * a.kchunk;
-        const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-        edge_march<K, G27>(a, cb * kBlock + threadIdx.x, k0, k1, a_s, epsv);
+    const int k0 = 1 + zc * a.kchunk;
+    const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+    if (r < ebx) {
+        const int e = r * 8 + x;
+        if (e < a.edge_blocks) edge_march<K, G27>(a, e * kBlock + threadIdx.x, k0, k1, a_s, epsv);
     } else {
-        const int ntile = a.tiles_x * a.tiles_y;
-        const int lb2 = lb - edge_total;
-        const int zc = lb2 / ntile;
-        const int r = lb2 - zc * ntile;
-        const int ty = r / a.tiles_x;
-        const int tx = r - ty * a.tiles_x;
-        const int k0 = 1 + zc * a.kchunk;
-        const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-        tile_march<K, G27>(a, 1 + tx * kTX, 1 + ty * kTY, k0, k1, a_s, epsv);
+        const int t = x * tpx + (r - ebx);
+        if (t >= ntile) return;
+        const int ty = t / a.tiles_x;
+        const int tx = t - ty * a.tiles_x;
+        tile_march<K, G27>(a, tx, ty, k0, k1, a_s, epsv);
     }
 }
 
@@ -1212,10 +1280,34 @@ __global__ __launch_bounds__(kBlock) void nucl_kernel(NuclArgs a)
                     const float v = a.ball[q++];
                     a.ph[o] = v;
                     if (v > -0.9999f) a.arg[o] = id;
+                    a.act[(gk + lk) * a.ntiles + act_tile_x(gi + li) + act_tile_y(gj + lj) * a.ntx] =
+                        a.ep;
                 }
         a.status[C] = 1;
         atomicAdd(a.events, 1ull);
     }
+}
+
+// Activity map from a buffer's contents (after it was written other than by
+// a step): one lane per interior point, K slots each.
+__global__ __launch_bounds__(kBlock) void act_rebuild_kernel(const float* __restrict__ ph,
+                                                             const int* __restrict__ arg,
+                                                             unsigned* act, unsigned ep, int fnx,
+                                                             int fny, int fnz, int K, int ntx,
+                                                             int ntiles)
+{
+    const int nx = fnx - 2, ny = fny - 2;
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= (long long)nx * ny * (fnz - 2)) return;
+    const int k = 1 + (int)(q / ((long long)nx * ny));
+    const int r = (int)(q - (long long)(k - 1) * nx * ny);
+    const int j = 1 + r / nx;
+    const int i = 1 + r - (j - 1) * nx;
+    const int L = fnx * fny * fnz;
+    const int C = i + j * fnx + k * fnx * fny;
+    bool on = false;
+    for (int s = 0; s < K; ++s) on = on || arg[C + s * L] != -1 || ph[C + s * L] != -1.0f;
+    if (on) act[k * ntiles + act_tile_x(i) + act_tile_y(j) * ntx] = ep;
 }
 
 // Plain streaming copy (16 B per lane per iteration), the reference point for
@@ -1235,7 +1327,7 @@ inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,
                        hipEvent_t ev_begin, hipEvent_t ev_mid, hipEvent_t ev_end)
 {
-    const int nblocks = (a.tiles_x * a.tiles_y + a.edge_blocks) * a.nzchunk;
+    const int nblocks = 8 * ((a.edge_blocks + 7) / 8 + (a.tiles_x * a.tiles_y + 7) / 8) * a.nzchunk;
     dim3 grid(nblocks), block(kBlock), fgrid(kFixupBlocks);
 #define CUPF_LAUNCH(KK, GG)                                                                   \
     do {                                                                                      \
@@ -1255,6 +1347,15 @@ hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t strea
         else return hipErrorInvalidValue;
     }
 #undef CUPF_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_act_rebuild(const float* ph, const int* arg, unsigned* act, unsigned ep, int fnx,
+                              int fny, int fnz, int K, int ntx, int ntiles, hipStream_t stream)
+{
+    const long long n = (long long)(fnx - 2) * (fny - 2) * (fnz - 2);
+    hipLaunchKernelGGL(act_rebuild_kernel, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, stream, ph, arg,
+                       act, ep, fnx, fny, fnz, K, ntx, ntiles);
     return hipGetLastError();
 }
 

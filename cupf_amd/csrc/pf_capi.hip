@@ -47,7 +47,14 @@ struct pf_ctx {
     float* ball = nullptr;
     int cnx = 0, cny = 0, cnz = 0;
     unsigned long long ncall = 0;
-    int kchunk = 32;                // z planes per workgroup march
+    int kchunk = 8;                 // z planes per workgroup march
+    // Activity map per buffer (one word per 64x4 column tile and z plane) and
+    // the epoch each buffer was last written at; see StepArgs.
+    unsigned* act[2] = {nullptr, nullptr};
+    unsigned ep[2] = {1u, 1u};
+    unsigned ep_ctr = 1u;
+    int use_act = 1;
+    int ntx = 0, nty = 0, ntiles = 0;
     bool have_coords = false, have_orient = false, have_state = false;
 };
 
@@ -117,9 +124,17 @@ StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
     a.ncols = (p.fnx - 2) * (p.fny - 2);
     a.kchunk = c->kchunk;
     a.nzchunk = (p.fnz - 2 + a.kchunk - 1) / a.kchunk;
-    a.nblk_col = (a.ncols + 255) / 256;
     a.tiles_x = (p.fnx - 2) / 64;
     a.tiles_y = (p.fny - 2) / 4;
+    a.act_src = c->act[src];
+    a.act_dst = c->act[1 - src];
+    a.ep_src = c->ep[src];
+    a.ep_dst = c->ep[1 - src];
+    a.ep_new = ++c->ep_ctr;
+    a.use_act = c->use_act;
+    a.ntx = c->ntx;
+    a.nty = c->nty;
+    a.ntiles = c->ntiles;
     {
         const int ex = (p.fnx - 2) - a.tiles_x * 64;
         const int ey = (p.fny - 2) - a.tiles_y * 4;
@@ -177,6 +192,10 @@ int nucl_launch(pf_ctx* c, float t, float dt_nuc, int use_iuc)
     a.T3d = c->T3d;
     a.ball = c->ball;
     a.events = c->counters + 1;
+    a.act = c->act[c->cur];
+    a.ep = c->ep[c->cur];
+    a.ntx = c->ntx;
+    a.ntiles = c->ntiles;
     a.fnx = p.fnx;
     a.fny = p.fny;
     a.fnz = p.fnz;
@@ -213,13 +232,35 @@ int nucl_launch(pf_ctx* c, float t, float dt_nuc, int use_iuc)
     return PF_OK;
 }
 
+// Activity map of buffer b from its contents, under a fresh epoch.
+hipError_t act_rebuild(pf_ctx* c, int b)
+{
+    const pf_params& p = c->p;
+    c->ep[b] = ++c->ep_ctr;
+    return launch_act_rebuild(c->ph[b], c->arg[b], c->act[b], c->ep[b], p.fnx, p.fny, p.fnz, p.K,
+                              c->ntx, c->ntiles, c->stream);
+}
+
+// Before the epoch counter could wrap: clear both maps and rebuild them.
+hipError_t act_renew_if_needed(pf_ctx* c)
+{
+    if (c->ep_ctr < 0x7fffffffu) return hipSuccess;
+    const size_t n = (size_t)c->ntiles * c->p.fnz;
+    hipError_t e;
+    for (int b = 0; b < 2; ++b)
+        if ((e = hipMemsetAsync(c->act[b], 0, n * sizeof(unsigned), c->stream)) != hipSuccess) return e;
+    c->ep_ctr = 1u;
+    if ((e = act_rebuild(c, 0)) != hipSuccess) return e;
+    return act_rebuild(c, 1);
+}
+
 void free_ctx(pf_ctx* c)
 {
     (void)hipSetDevice(c->device);
     void* bufs[] = {c->ph[0], c->ph[1], c->arg[0], c->arg[1], c->x, c->y, c->z, c->zbuf,
                     c->R12, c->psi_mac, c->T3d, c->phi0, c->phi, c->alpha, c->counters,
                     c->sums, c->alpha_full, c->loss, c->nstatus, c->ball,
-                    c->fix_list, c->fix_count, c->sink};
+                    c->fix_list, c->fix_count, c->sink, c->act[0], c->act[1]};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -262,8 +303,12 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
     c->p = p;
     if (const char* kc = getenv("CUPF_KCHUNK")) {
         const int v = atoi(kc);
-        if (v >= 1 && v <= 4096) c->kchunk = v;
+        if (v >= 1 && v <= 61) c->kchunk = v;  // plane masks of a chunk fit 64 bits
     }
+    if (const char* na = getenv("CUPF_NO_ACTIVITY")) c->use_act = atoi(na) ? 0 : 1;
+    c->ntx = (p.fnx - 2 + 63) / 64;
+    c->nty = (p.fny - 2 + 3) / 4;
+    c->ntiles = c->ntx * c->nty;
     c->device = device;
     c->L = (int)L;
     int rc = PF_OK;
@@ -284,7 +329,9 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         (rc = dalloc(&c->phi, (size_t)L)) || (rc = dalloc(&c->alpha, (size_t)L)) ||
         (rc = dalloc(&c->counters, 2)) || (rc = dalloc(&c->sums, p.fnz)) ||
         (rc = dalloc(&c->fix_list, (size_t)(p.fnx - 2) * (p.fny - 2) * (p.fnz - 2))) ||
-        (rc = dalloc(&c->fix_count, 2)) || (rc = dalloc(&c->sink, 128)))
+        (rc = dalloc(&c->fix_count, 2)) || (rc = dalloc(&c->sink, 128)) ||
+        (rc = dalloc(&c->act[0], (size_t)c->ntiles * p.fnz)) ||
+        (rc = dalloc(&c->act[1], (size_t)c->ntiles * p.fnz)))
         return bail(rc);
     c->h_sums.resize(p.fnz);
     for (int b = 0; b < 2; ++b) {
@@ -295,6 +342,8 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
     if (hipMemsetAsync(c->counters, 0, 2 * sizeof(unsigned long long), c->stream) != hipSuccess ||
         hipMemsetAsync(c->fix_count, 0, 2 * sizeof(unsigned), c->stream) != hipSuccess ||
         hipMemsetAsync(c->alpha, 0, L * sizeof(int), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->act[0], 0, (size_t)c->ntiles * p.fnz * sizeof(unsigned), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->act[1], 0, (size_t)c->ntiles * p.fnz * sizeof(unsigned), c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return bail(fail(PF_E_HIP, "initial fill failed"));
     *out = c;
@@ -403,6 +452,8 @@ int pf_init_from_phi_alpha(pf_ctx* c, const float* phi0, const int* alpha0)
     c->cur = 0;
     HIPCHK(launch_ini(c->ph[0], c->arg[0], c->ph[1], c->arg[1], c->phi0, c->alpha, c->L, c->p.K,
                       c->stream));
+    HIPCHK(act_rebuild(c, 0));
+    HIPCHK(act_rebuild(c, 1));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_state = true;
     return PF_OK;
@@ -416,6 +467,8 @@ int pf_reinit_from_collect(pf_ctx* c)
     c->cur = 0;
     HIPCHK(launch_ini(c->ph[0], c->arg[0], c->ph[1], c->arg[1], c->phi0, c->alpha, c->L, c->p.K,
                       c->stream));
+    HIPCHK(act_rebuild(c, 0));
+    HIPCHK(act_rebuild(c, 1));
     HIPCHK(hipStreamSynchronize(c->stream));
     return PF_OK;
 }
@@ -429,6 +482,7 @@ int pf_upload_state(pf_ctx* c, int which, const float* ph, const int* arg)
     const size_t n = (size_t)c->L * c->p.K;
     HIPCHK(hipMemcpyAsync(c->ph[b], ph, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->arg[b], arg, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(act_rebuild(c, b));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_state = true;
     return PF_OK;
@@ -471,8 +525,10 @@ static int step_impl(pf_ctx* c, int nsteps, int first_step, int flags)
     for (int s = first_step; s < first_step + nsteps; ++s) {
         // t_cur_step = (2*kt+1)*dt*tau0 in float (3DgrainMPI/APTPhaseField.cu:863,919).
         const float t = s * p.dt * p.tau0;
+        HIPCHK(act_renew_if_needed(c));
         StepArgs a = make_step_args(c, c->cur, t, use_iuc);
         HIPCHK(launch_step(a, p.K, p.gather27, c->stream));
+        c->ep[1 - c->cur] = a.ep_new;
         c->cur = 1 - c->cur;
         if (nucl && (s & 1)) {
             int rc = nucl_launch(c, t, 2.0f * p.dt * p.tau0, use_iuc);
@@ -520,8 +576,11 @@ int pf_step_profiled(pf_ctx* c, int nsteps, int first_step, int flags, float* ms
     for (int q = 0; q < nsteps && err == hipSuccess; ++q) {
         const int s = first_step + q;
         const float t = s * p.dt * p.tau0;
+        err = act_renew_if_needed(c);
+        if (err != hipSuccess) break;
         StepArgs a = make_step_args(c, c->cur, t, use_iuc);
         err = launch_step(a, p.K, p.gather27, c->stream, ev[3 * q], ev[3 * q + 1], ev[3 * q + 2]);
+        c->ep[1 - c->cur] = a.ep_new;
         c->cur = 1 - c->cur;
     }
     if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
@@ -616,6 +675,9 @@ int pf_md_shift(pf_ctx* c, int move_count)
     HIPCHK(launch_move_frame(c->ph[b], c->arg[b], c->ph[1 - b], c->arg[1 - b], c->z, c->zbuf,
                              c->alpha, c->alpha_full, c->loss, move_count, p.fnx, p.fny, p.fnz,
                              p.K, p.num_theta, c->stream));
+    // both buffers were rewritten (the other one served as scratch)
+    HIPCHK(act_rebuild(c, b));
+    HIPCHK(act_rebuild(c, 1 - b));
     HIPCHK(hipStreamSynchronize(c->stream));
     return PF_OK;
 }
@@ -735,6 +797,29 @@ int pf_copy_bandwidth(int device, unsigned long long bytes, int iters, float* gb
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);
     return rc;
+}
+
+int pf_activity(pf_ctx* c, int which, unsigned char* map, long long* active, long long* total)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    const int b = buf_index(c, which);
+    const pf_params& p = c->p;
+    std::vector<unsigned> h((size_t)c->ntiles * p.fnz);
+    HIPCHK(hipMemcpyAsync(h.data(), c->act[b], sizeof(unsigned) * h.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    long long n = 0;
+    for (int k = 0; k < p.fnz; ++k)
+        for (int t = 0; t < c->ntiles; ++t) {
+            const size_t q = (size_t)k * c->ntiles + t;
+            const bool on = k > 0 && k < p.fnz - 1 && h[q] >= c->ep[b];
+            if (map) map[q] = on ? 1 : 0;
+            n += on;
+        }
+    if (active) *active = n;
+    if (total) *total = (long long)c->ntiles * (p.fnz - 2);
+    return PF_OK;
 }
 
 int pf_sync(pf_ctx* c)

@@ -176,6 +176,17 @@ int pf_overflow_count(pf_ctx* ctx, long long* count);
 /* Wait for all work on the context's stream. */
 int pf_sync(pf_ctx* ctx);
 
+/* Diagnostic: activity map of buffer `which` (0 current, 1 other).  The step
+ * skips z planes of 64x4 column tiles whose stencil reaches only empty slot
+ * pairs (-1, -1.0) and whose destination is empty too.  Tiles are numbered
+ * t = tx + ty*ceil((fnx-2)/64), tx = (i-1)/64, ty = (j-1)/4 over interior
+ * columns.  map (nullable, fnz*ntiles bytes, index k*ntiles + t) receives 1
+ * for a tile plane that may hold a non-empty slot; *active their number,
+ * *total all interior tile planes.  No counterpart in the reference, which
+ * updates every point; results are identical either way.  Environment
+ * variable CUPF_NO_ACTIVITY=1 at pf_create turns the skipping off. */
+int pf_activity(pf_ctx* ctx, int which, unsigned char* map, long long* active, long long* total);
+
 /* Diagnostic: bandwidth of a plain device-to-device copy kernel over `bytes`
  * (read + write counted), iters launches; the reference point for the HBM
  * roofline and for scaling HBM byte counters.  No counterpart in the reference. */
