@@ -97,7 +97,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=400)
     ap.add_argument("--K", type=int, default=8)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=80)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-only", type=int, default=0,
                     help="only time a plain copy kernel over this many bytes (counter calibration)")

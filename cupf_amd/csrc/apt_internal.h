@@ -27,7 +27,11 @@ struct StepArgs {
     const float* psi_mac;  // type-4 thermal field
     unsigned long long* overflow;  // count of points with > K distinct ids
     unsigned* fix_list;    // interior points left to the fix-up pass (capacity: interior points)
-    unsigned* fix_count;   // [0] entries in fix_list, [1] fix-up workgroups finished
+    // Counters of this step: fix_count[0] entries in fix_list; wl_count (below)
+    // points into the same 4-word set.  Consecutive steps alternate between two
+    // sets; cnt_next is the other one, cleared by this step's plan kernel.
+    unsigned* fix_count;
+    unsigned* cnt_next;
     int* sink_i;           // 64-lane write sinks for lanes that must not store
     float* sink_f;
     // Activity map: one word per (column tile, z plane) and buffer.  A tile
@@ -41,12 +45,24 @@ struct StepArgs {
     unsigned ep_new;       // epoch the destination buffer gets from this step
     int use_act;           // 0: process every plane (the map is still maintained)
     int ntx, nty, ntiles;  // activity tiles (64x4, partial ones included)
+    // Work of one step: nzchunk * edge_items edge items (256 points of the
+    // leftover columns each), then the tile chunks listed by the plan kernel:
+    // wl_item = z chunk * items_chunk + full tile, with the chunk's plane masks.
+    // wl_count[0] listed tile chunks, wl_count[1] the next work index to take
+    // (words 2 and 3 of this step's counter set).
+    int* wl_item;
+    unsigned long long* wl_need;
+    unsigned long long* wl_act;
+    unsigned* wl_count;
+    int items_chunk;       // full tiles per z chunk
+    int edge_cols;         // interior columns outside the full tiles
+    int edge_items;        // edge items per z chunk: ceil(kchunk * edge_cols / 256)
+    int step_blocks;       // persistent workgroups of the step kernel
     int fnx, fny, fnz;
     int L;                 // slot stride fnx*fny*fnz
     int ncolx, ncols;      // interior columns: (fnx-2), (fnx-2)*(fny-2)
     int nzchunk, kchunk;   // z-chunks per column, planes per chunk
     int tiles_x, tiles_y;  // full 64x4 column tiles (LDS path)
-    int edge_blocks;       // 256-column blocks over the columns outside the tiles
     int bcx, bcy, bcz;
     int num_theta;
     int thermal_type;
@@ -86,6 +102,8 @@ struct NuclArgs {
 
 // Launchers (apt_kernels.hip).  All enqueue on `stream` and return the
 // hipError_t of the launch.
+// Workgroups of the persistent step kernel that fit the device at once.
+int step_kernel_slots(int K, int gather27, int device);
 // Main step kernel then the overflow fix-up kernel.  Optional events are
 // recorded before the main kernel, between the two and after the fix-up.
 hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,

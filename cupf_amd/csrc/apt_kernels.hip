@@ -98,13 +98,13 @@ __device__ __forceinline__ float interp4(const float* __restrict__ u, float x, f
     return (1.0f - dt) * up + dt * un;
 }
 
-// Thermal value of one point (3DgrainMPI/APTPhaseField.cu:278-327;
-// type 5: 3Dgrain/APTPhaseField.cu:303).  Returns true where the reference
-// `continue`s past the slot, leaving the destination untouched.
-__device__ __forceinline__ bool thermal_T(const StepArgs& a, int i, int j, int k, float& T)
+// Thermal value of one point at coordinates (xi, yj, zk)
+// (3DgrainMPI/APTPhaseField.cu:278-327; type 5: 3Dgrain/APTPhaseField.cu:303).
+// Returns true where the reference `continue`s past the slot, leaving the
+// destination untouched.
+__device__ __forceinline__ bool thermal_Tv(const StepArgs& a, float xi, float yj, float zk, float& T)
 {
     const float t = a.t;
-    const float zk = a.z[k];
     if (a.use_iuc) {
         T = -a.uc0 * 1e6 * t;
         return false;
@@ -114,14 +114,12 @@ __device__ __forceinline__ bool thermal_T(const StepArgs& a, int i, int j, int k
         T = a.G * (zk - a.z0) - a.uc * 1e6 * t;
         return false;
     case 2: {
-        const float yj = a.y[j];
         float dist = sqrtf((yj - 0.5f * a.lyd) * (yj - 0.5f * a.lyd)
                            + (zk - a.z0 - a.lzd) * (zk - a.z0 - a.lzd)) - a.r0;
         T = -a.G * dist - a.uc * 1e6 * t;
         return false;
     }
     case 3: {
-        const float xi = a.x[i], yj = a.y[j];
         float lm = (a.r0 - a.z0) / a.tan_angle;
         float x_start = 0;
         float xl = xi - x_start;
@@ -134,7 +132,6 @@ __device__ __forceinline__ bool thermal_T(const StepArgs& a, int i, int j, int k
         return false;
     }
     case 4: {
-        const float xi = a.x[i], yj = a.y[j];
         float xq = xi - a.V * t * 1e6;
         if (xq > a.mp_len + a.x0 || xq < 0.0f) return true;
         float v = interp3(a.psi_mac, xq - a.X0_mac, yj - a.Y0_mac, zk - a.Z0_mac, a.Nx, a.Ny,
@@ -149,6 +146,15 @@ __device__ __forceinline__ bool thermal_T(const StepArgs& a, int i, int j, int k
         T = 0.0f;
         return false;
     }
+}
+
+// The same at grid point (i, j, k), coordinates read from the x/y/z arrays.
+__device__ __forceinline__ bool thermal_T(const StepArgs& a, int i, int j, int k, float& T)
+{
+    const int tt = a.thermal_type;
+    const bool nx = !a.use_iuc && (tt == 3 || tt == 4);
+    const bool ny = !a.use_iuc && (tt >= 2 && tt <= 4);
+    return thermal_Tv(a, nx ? a.x[i] : 0.0f, ny ? a.y[j] : 0.0f, a.z[k], T);
 }
 
 // Gather state of one point.  la: slot ids in first-appearance order (no
@@ -265,34 +271,54 @@ __device__ __forceinline__ void scan_all(const StepArgs& a, const Nbr& nb, Gathe
     }
 }
 
+// Orientation rows of a slot: the grain's rotation where its centre phi
+// exceeds LS (double compare), the identity elsewhere
+// (3DgrainMPI/APTPhaseField.cu:258-273).
+struct Rot {
+    float m0, m1, m2, m3, m4, m5, m6, m7, m8;
+};
+
+__device__ __forceinline__ Rot slot_rot(const StepArgs& a, int gid, float phC)
+{
+    Rot r{1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+    if ((double)phC > kLS) {
+        const int tid = gid <= a.num_theta ? gid : (gid % a.num_theta) + 1;
+        const float4* p = reinterpret_cast<const float4*>(a.R12 + 12 * tid);
+        const float4 r0 = p[0], r1 = p[1], r2 = p[2];
+        r = Rot{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x};
+    }
+    return r;
+}
+
 // Right-hand side and explicit Euler update of one filled slot
-// (3DgrainMPI/APTPhaseField.cu:250-340).  repul is the ascending sum of
-// 0.25*(phiC_p+1)^2 over the other filled slots (empty ones add exactly +0).
-__device__ __forceinline__ float slot_update(const StepArgs& a, int gid, float phD, float phB,
-                                             float phL, float phC, float phR, float phT,
-                                             float phU, float repul, float Up, float a_s,
-                                             float epsv)
+// (3DgrainMPI/APTPhaseField.cu:250-340) with its orientation rows.  repul is
+// the ascending sum of 0.25*(phiC_p+1)^2 over the other filled slots (empty
+// ones add exactly +0).
+__device__ __forceinline__ float slot_update_r(const StepArgs& a, const Rot& m, float phD,
+                                               float phB, float phL, float phC, float phR,
+                                               float phT, float phU, float repul, float Up,
+                                               float a_s, float epsv)
 {
     const float phxn = (phR - phL) * 0.5f;
     const float phyn = (phT - phB) * 0.5f;
     const float phzn = (phU - phD) * 0.5f;
-    float m0 = 1.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f, m4 = 1.0f, m5 = 0.0f, m6 = 0.0f, m7 = 0.0f,
-          m8 = 1.0f;
-    if ((double)phC > kLS) {
-        const int tid = gid <= a.num_theta ? gid : (gid % a.num_theta) + 1;
-        const float4* r = reinterpret_cast<const float4*>(a.R12 + 12 * tid);
-        const float4 r0 = r[0], r1 = r[1], r2 = r[2];
-        m0 = r0.x; m1 = r0.y; m2 = r0.z; m3 = r0.w;
-        m4 = r1.x; m5 = r1.y; m6 = r1.z; m7 = r1.w;
-        m8 = r2.x;
-    }
-    const float A2 = kine_ani(phxn, phyn, phzn, m0, m1, m2, m3, m4, m5, m6, m7, m8, a_s, epsv, a.eps);
+    const float A2 = kine_ani(phxn, phyn, phzn, m.m0, m.m1, m.m2, m.m3, m.m4, m.m5, m.m6, m.m7,
+                              m.m8, a_s, epsv, a.eps);
     const float diff = phR + phL + phT + phB + phU + phD - 6.0f * phC;
     const float omc = 1.0f - phC * phC;
     const float rhs = diff * a.hi * a.hi + omc * phC
                     - a.lamd * Up * (omc * omc - 6.0f * (phC + 1.0f) * repul);
     const float dphi = rhs / A2;
     return phC + a.dt * dphi;
+}
+
+__device__ __forceinline__ float slot_update(const StepArgs& a, int gid, float phD, float phB,
+                                             float phL, float phC, float phR, float phT,
+                                             float phU, float repul, float Up, float a_s,
+                                             float epsv)
+{
+    return slot_update_r(a, slot_rot(a, gid, phC), phD, phB, phL, phC, phR, phT, phU, repul, Up,
+                         a_s, epsv);
 }
 
 // Pass B: right-hand side and explicit Euler update of every slot of one point
@@ -359,14 +385,40 @@ __device__ __forceinline__ void rhs_point(const StepArgs& a, int i, int j, int k
     }
 }
 
-// Pass B into registers: out_id/out_ph hold the new slot values and wr is
-// false only for slots the reference leaves untouched (the thermal
-// `continue`).  Lets the caller issue one straight-line block of stores.
+// Pass B into registers, in two phases so that the memory reads of a point
+// (orientation rows, the paraboloid field) can be issued before the next
+// plane's prefetch: slot_prep evaluates the thermal field, the centre phi of
+// every slot and the rows; slot_finish computes the new slot values into
+// out_id/out_ph.  wr is false only for slots the reference leaves untouched
+// (the thermal `continue`).  The caller then issues one straight-line block
+// of stores.
+template <int K>
+struct SlotPrep {
+    float pc[K];
+    float Up;
+    bool skip;
+};
+
 template <int K, class PhiAt>
-__device__ __forceinline__ void rhs_point_regs(const StepArgs& a, int i, int j, int k,
-                                               const Gather<K>& g, float a_s, float epsv,
-                                               PhiAt&& phi_at, int (&out_id)[K],
-                                               float (&out_ph)[K], bool& wr)
+__device__ __forceinline__ void slot_prep(const StepArgs& a, const Gather<K>& g, float xi, float yj,
+                                          float zk, PhiAt&& phi_at, SlotPrep<K>& sp)
+{
+    float T;
+    sp.skip = thermal_Tv(a, xi, yj, zk, T);
+    sp.Up = T / a.L_cp;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const uint32_t p = (g.pm[3] >> (4 * q)) & 15u;
+        sp.pc[q] = (q < g.n && p != 15u) ? phi_at(3, p) : -1.0f;
+
+    }
+}
+
+template <int K, class PhiAt>
+__device__ __forceinline__ void slot_finish(const StepArgs& a, const Gather<K>& g,
+                                            const SlotPrep<K>& sp, float a_s, float epsv,
+                                            PhiAt&& phi_at, int (&out_id)[K], float (&out_ph)[K],
+                                            bool& wr)
 {
 #pragma unroll
     for (int q = 0; q < K; ++q) {
@@ -375,17 +427,8 @@ __device__ __forceinline__ void rhs_point_regs(const StepArgs& a, int i, int j, 
     }
     wr = true;
     if (g.n == 0) return;
-    float T;
-    const bool skip = thermal_T(a, i, j, k, T);
-    wr = !skip;
-    if (skip) return;
-    const float Up = T / a.L_cp;
-    float pc[K];
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-        const uint32_t p = (g.pm[3] >> (4 * q)) & 15u;
-        pc[q] = (q < g.n && p != 15u) ? phi_at(3, p) : -1.0f;
-    }
+    wr = !sp.skip;
+    if (sp.skip) return;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         if (q >= g.n) continue;
@@ -393,7 +436,7 @@ __device__ __forceinline__ void rhs_point_regs(const StepArgs& a, int i, int j, 
 #pragma unroll
         for (int s = 0; s < 7; ++s) {
             if (s == 3) {
-                v[s] = pc[q];
+                v[s] = sp.pc[q];
             } else {
                 const uint32_t p = (g.pm[s] >> (4 * q)) & 15u;
                 v[s] = (p != 15u) ? phi_at(s, p) : -1.0f;
@@ -404,12 +447,12 @@ __device__ __forceinline__ void rhs_point_regs(const StepArgs& a, int i, int j, 
 #pragma unroll
         for (int p = 0; p < K; ++p) {
             if (p != q && p < g.n) {
-                const float w = pc[p] + 1.0f;
+                const float w = sp.pc[p] + 1.0f;
                 repul += 0.25f * w * w;
             }
         }
-        const float nv = slot_update(a, gid, v[0], v[1], v[2], v[3], v[4], v[5], v[6], repul, Up,
-                                     a_s, epsv);
+        const float nv = slot_update_r(a, slot_rot(a, gid, sp.pc[q]), v[0], v[1], v[2], v[3], v[4], v[5], v[6], repul,
+                                       sp.Up, a_s, epsv);
         out_ph[q] = nv;
         out_id[q] = ((double)nv < kKill) ? -1 : gid;
     }
@@ -643,6 +686,7 @@ struct PlaneLoader {
     int off[PER];                                         // global offset minus the plane term
     int vid[PER];
     float vph[PER];
+    bool live;                                            // staged plane holds non-empty slots
 };
 
 template <int K>
@@ -664,23 +708,19 @@ __device__ __forceinline__ void loader_init(PlaneLoader<K>& ld, const StepArgs& 
 
 // Elements 0..PER-2 of every lane are always inside the plane; only the last
 // one can fall past NE, and its offset is clamped so the load stays in bounds
-// (the value is then simply not stored).  An inactive plane (all empty pairs
-// in the source) is staged without reading memory.
+// (the value is then simply not stored).  The loads are issued whatever the
+// plane: `fetch` false (nothing will be staged) and `active` false (all empty
+// pairs in the source, staged as (-1, -1.0f)) read element 0 instead, so the
+// number of vector-memory operations in flight is the same on every path.
 template <int K>
 __device__ __forceinline__ void loader_fetch(PlaneLoader<K>& ld, const StepArgs& a, int plane,
-                                             bool active)
+                                             bool active, bool fetch = true)
 {
-    if (!active) {
-#pragma unroll
-        for (int q = 0; q < PlaneLoader<K>::PER; ++q) {
-            ld.vid[q] = -1;
-            ld.vph[q] = -1.0f;
-        }
-        return;
-    }
+    const bool real = active && fetch;
+    ld.live = active;
 #pragma unroll
     for (int q = 0; q < PlaneLoader<K>::PER; ++q) {
-        const int o = (q == PlaneLoader<K>::PER - 1 && ld.off[q] < 0) ? 0 : ld.off[q] + plane;
+        const int o = (!real || (q == PlaneLoader<K>::PER - 1 && ld.off[q] < 0)) ? 0 : ld.off[q] + plane;
         ld.vid[q] = a.arg[o];
         ld.vph[q] = a.ph[o];
     }
@@ -692,8 +732,8 @@ __device__ __forceinline__ void loader_store(const PlaneLoader<K>& ld, int* ids,
 #pragma unroll
     for (int q = 0; q < PlaneLoader<K>::PER; ++q) {
         if (q < PlaneLoader<K>::PER - 1 || ld.off[q] >= 0) {
-            ids[threadIdx.x + q * kBlock] = ld.vid[q];
-            phs[threadIdx.x + q * kBlock] = ld.vph[q];
+            ids[threadIdx.x + q * kBlock] = ld.live ? ld.vid[q] : -1;
+            phs[threadIdx.x + q * kBlock] = ld.live ? ld.vph[q] : -1.0f;
         }
     }
 }
@@ -764,55 +804,52 @@ __device__ __forceinline__ void fixup_append(const StepArgs& a, int ovf, unsigne
     if (ovf) a.fix_list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = point;
 }
 
-// One z plane of a tile: stage the prefetched plane, gather from LDS, compute
-// every slot into registers, then one straight-line block of 2K stores.  With
-// no vector-memory traffic in the branchy compute part, the wait for the
-// prefetched loads at the top of the next iteration only has to cover those
-// loads, not the stores issued after them.  `pf_next` says whether plane k+2
-// is fetched (only when plane k+1 is processed too); `act_next2` whether that
-// plane holds anything in the source.
+// One z plane of a tile.  On entry the ring holds planes k-1, k, k+1.
+// Order of the vector-memory work: gather from LDS, then the point's own
+// reads (orientation rows, paraboloid field), then the prefetch of plane k+2
+// (when plane k+1 is processed next), then the slot arithmetic, then one
+// straight-line block of 2K stores, then plane k+2 is staged into the ring
+// slot plane k-1 leaves free.  Vector-memory operations retire in issue
+// order, so the arithmetic only waits for the point's reads, never for the
+// prefetch, and staging only for the prefetch, never for the stores; every
+// path into an iteration has the same operations in flight.
 template <int K, bool G27>
 __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld, int (*rid)[K * kNPT],
-                                           float (*rph)[K * kNPT], int i, int j, int k, int tile,
-                                           int pt, float a_s, float epsv, bool pf_next,
-                                           bool act_next2)
+                                           float (*rph)[K * kNPT], const float* zs, int i, int j,
+                                           int k, int k0, int tile, int pt, float xi, float yj,
+                                           float a_s, float epsv, bool stage_next, bool act_next2)
 {
     const int A = a.fnx * a.fny;
-    loader_store(ld, rid[(k + 1) % 3], rph[(k + 1) % 3]);
     __syncthreads();
-    if (pf_next) loader_fetch(ld, a, bc_map(k + 2, a.fnz, a.bcz) * A, act_next2);
     const int sD = (k + 2) % 3, sC = k % 3, sU = (k + 1) % 3;
     Gather<K> g;
     gather_init(g);
     scan_all_lds<K, G27>(rid[sD], rid[sC], rid[sU], pt, g);
     const int ovf = g.ovf ? 1 : 0;
     const int C = i + j * a.fnx + k * A;
+    const float* pD = rph[sD];
+    const float* pC = rph[sC];
+    const float* pU = rph[sU];
+    auto phi_at = [&](int s, uint32_t p) {
+        const int o = (int)p * kNPT;
+        switch (s) {
+        case 0: return pD[o + pt];
+        case 1: return pC[o + pt - kHX];
+        case 2: return pC[o + pt - 1];
+        case 3: return pC[o + pt];
+        case 4: return pC[o + pt + 1];
+        case 5: return pC[o + pt + kHX];
+        default: return pU[o + pt];
+        }
+    };
+    SlotPrep<K> sp;
+    if (ovf) g.n = 0;  // an overflowed point computes nothing here (fix-up kernel)
+    slot_prep<K>(a, g, xi, yj, zs[k - k0], phi_at, sp);
+    loader_fetch(ld, a, bc_map(k + 2, a.fnz, a.bcz) * A, act_next2, stage_next);
     int oid[K];
     float oph[K];
-    bool wr = true;
-    if (!ovf) {
-        const float* pD = rph[sD];
-        const float* pC = rph[sC];
-        const float* pU = rph[sU];
-        rhs_point_regs<K>(a, i, j, k, g, a_s, epsv, [&](int s, uint32_t p) {
-            const int o = (int)p * kNPT;
-            switch (s) {
-            case 0: return pD[o + pt];
-            case 1: return pC[o + pt - kHX];
-            case 2: return pC[o + pt - 1];
-            case 3: return pC[o + pt];
-            case 4: return pC[o + pt + 1];
-            case 5: return pC[o + pt + kHX];
-            default: return pU[o + pt];
-            }
-        }, oid, oph, wr);
-    } else {
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            oid[q] = -1;
-            oph[q] = -1.0f;
-        }
-    }
+    bool wr;
+    slot_finish<K>(a, g, sp, a_s, epsv, phi_at, oid, oph, wr);
     // Overflowed points are written by the fix-up kernel; skipped slots keep the
     // destination's old value.  Both send their stores to the per-lane sink.
     const int lane = threadIdx.x & 63;
@@ -828,18 +865,19 @@ __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld
     // will, through the fix-up): mark the destination tile plane.
     if (ovf || g.n > 0) a.act_dst[k * a.ntiles + tile] = a.ep_new;
     fixup_append(a, ovf, (unsigned)C);
-    __syncthreads();
+    if (stage_next) {
+        __syncthreads();
+        loader_store(ld, rid[sD], rph[sD]);
+    }
 }
 
-// Planes of the chunk [k0, k1) to process, bit m for plane k0+m, and the
-// source activity of planes k0-1 .. k1, bit r for plane k0-1+r (through the
-// z boundary map).  Source words of the 3 x 3 tiles around (tx, ty) at the
-// mapped planes, destination words of the own tile.  Every thread of the
-// workgroup gets the same masks.
-__device__ __forceinline__ void tile_need(const StepArgs& a, int tx, int ty, int i0, int j0, int k0,
-                                          int k1, uint64_t& need, uint64_t& act)
+// Planes of the chunk [k0, k1) of tile (tx, ty) to process, bit m for plane
+// k0+m, and the source activity of planes k0-1 .. k1, bit r for plane k0-1+r
+// (through the z boundary map): source words of the 3 x 3 tiles around the
+// tile at the mapped planes, destination words of the tile itself.
+__device__ __forceinline__ void chunk_masks(const StepArgs& a, int tx, int ty, int k0, int k1,
+                                            uint64_t& need, uint64_t& act)
 {
-    __shared__ unsigned long long s_act, s_dst;
     const int n = k1 - k0;
     const uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
     if (!a.use_act) {
@@ -847,39 +885,32 @@ __device__ __forceinline__ void tile_need(const StepArgs& a, int tx, int ty, int
         act = ~0ull;
         return;
     }
-    if (threadIdx.x == 0) {
-        s_act = 0ull;
-        s_dst = 0ull;
-    }
-    __syncthreads();
+    const int i0 = 1 + tx * kTX, j0 = 1 + ty * kTY;
     const int xl = act_tile_x(bc_map(i0 - 1, a.fnx, a.bcx)), xr = act_tile_x(bc_map(i0 + kTX, a.fnx, a.bcx));
     const int yl = act_tile_y(bc_map(j0 - 1, a.fny, a.bcy)), yr = act_tile_y(bc_map(j0 + kTY, a.fny, a.bcy));
-    for (int t = threadIdx.x; t < 9 * (n + 2); t += kBlock) {
-        const int r = t / 9, q = t - 9 * r;
-        const int qx = q % 3, qy = q / 3;
-        const int ttx = qx == 0 ? xl : (qx == 1 ? tx : xr);
-        const int tty = qy == 0 ? yl : (qy == 1 ? ty : yr);
-        const int kk = bc_map(k0 - 1 + r, a.fnz, a.bcz);
-        if (a.act_src[kk * a.ntiles + ttx + tty * a.ntx] >= a.ep_src) atomicOr(&s_act, 1ull << r);
+    const int rl = yl * a.ntx, rc = ty * a.ntx, rr = yr * a.ntx;
+    uint64_t ac = 0ull, dst = 0ull;
+    for (int r = 0; r < n + 2; ++r) {
+        const unsigned* w = a.act_src + bc_map(k0 - 1 + r, a.fnz, a.bcz) * a.ntiles;
+        const unsigned e = a.ep_src;
+        const bool on = w[xl + rl] >= e || w[tx + rl] >= e || w[xr + rl] >= e ||
+                        w[xl + rc] >= e || w[tx + rc] >= e || w[xr + rc] >= e ||
+                        w[xl + rr] >= e || w[tx + rr] >= e || w[xr + rr] >= e;
+        if (on) ac |= 1ull << r;
+        if (r < n && a.act_dst[(k0 + r) * a.ntiles + tx + rc] >= a.ep_dst) dst |= 1ull << r;
     }
-    const int own = tx + ty * a.ntx;
-    if ((int)threadIdx.x < n && a.act_dst[(k0 + threadIdx.x) * a.ntiles + own] >= a.ep_dst)
-        atomicOr(&s_dst, 1ull << threadIdx.x);
-    __syncthreads();
-    act = s_act;
-    need = ((act | (act >> 1) | (act >> 2)) | s_dst) & full;
+    act = ac;
+    need = ((ac | (ac >> 1) | (ac >> 2)) | dst) & full;
 }
 
 template <int K, bool G27>
 __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, int k0, int k1,
-                                           float a_s, float epsv)
+                                           uint64_t need, uint64_t act, float a_s, float epsv)
 {
     __shared__ int rid[3][K * kNPT];
     __shared__ float rph[3][K * kNPT];
+    __shared__ float zs[64];  // z of the chunk's planes (a chunk has at most 61)
     const int i0 = 1 + tx * kTX, j0 = 1 + ty * kTY;
-    uint64_t need, act;
-    tile_need(a, tx, ty, i0, j0, k0, k1, need, act);
-    if (need == 0ull) return;
     const int lx = threadIdx.x & (kTX - 1);
     const int ly = threadIdx.x / kTX;
     const int i = i0 + lx, j = j0 + ly;
@@ -888,6 +919,10 @@ __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, in
     const int tile = tx + ty * a.ntx;
     auto act_at = [&](int kk) { return kk <= k1 && ((act >> (kk - k0 + 1)) & 1ull) != 0ull; };
     auto need_at = [&](int kk) { return kk < k1 && ((need >> (kk - k0)) & 1ull) != 0ull; };
+    // coordinates for the thermal field: per lane once, z through LDS (made
+    // visible by the barrier that opens every run)
+    const float xi = a.x[i], yj = a.y[j];
+    if ((int)threadIdx.x < k1 - k0) zs[threadIdx.x] = a.z[k0 + threadIdx.x];
     PlaneLoader<K> ld;
     loader_init(ld, a, i0, j0);
     int k = k0;
@@ -896,116 +931,163 @@ __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, in
             ++k;
             continue;
         }
-        // start of a run of processed planes: k-1 and k into the ring, k+1 into
-        // registers (the ring slots are free: the previous run ended with a barrier)
+        // start of a run of processed planes: k-1, k, k+1 into the ring once
+        // every wave is done with the ring
+        __syncthreads();
         loader_fetch(ld, a, bc_map(k - 1, a.fnz, a.bcz) * A, act_at(k - 1));
         loader_store(ld, rid[(k + 2) % 3], rph[(k + 2) % 3]);
         loader_fetch(ld, a, k * A, act_at(k));
         loader_store(ld, rid[k % 3], rph[k % 3]);
         loader_fetch(ld, a, bc_map(k + 1, a.fnz, a.bcz) * A, act_at(k + 1));
+        loader_store(ld, rid[(k + 1) % 3], rph[(k + 1) % 3]);
         do {
-            tile_plane<K, G27>(a, ld, rid, rph, i, j, k, tile, pt, a_s, epsv, need_at(k + 1),
-                               act_at(k + 2));
+            tile_plane<K, G27>(a, ld, rid, rph, zs, i, j, k, k0, tile, pt, xi, yj, a_s, epsv,
+                               need_at(k + 1), act_at(k + 2));
             ++k;
         } while (need_at(k));
     }
 }
 
 // Columns outside the full tiles (the strip left when fnx-2 or fny-2 is not
-// a multiple of the tile size) take the direct path, with the same plane
-// skipping per column.
+// a multiple of the tile size) take the direct path, one point per lane:
+// point p of z chunk [k0, k1) is edge column p % edge_cols at plane
+// k0 + p / edge_cols.  Edge columns are numbered leftover x columns first
+// (all rows), then leftover y rows under the full tiles.  A point is skipped
+// under the same rule as a tile plane: all of its stencil's tile planes empty
+// in the source and its own tile plane empty in the destination.
 template <int K, bool G27>
-__device__ __forceinline__ void edge_march(const StepArgs& a, int c, int k0, int k1, float a_s,
+__device__ __forceinline__ void edge_point(const StepArgs& a, int p, int k0, int k1, float a_s,
                                            float epsv)
 {
-    const int ex = (a.fnx - 2) - a.tiles_x * kTX;  // leftover columns at the +x end, all rows
-    const int A = a.fnx * a.fny;
+    const int r = p / a.edge_cols;
+    const int c = p - r * a.edge_cols;
+    const int k = k0 + r;
+    const int ex = (a.fnx - 2) - a.tiles_x * kTX;
     int i, j;
-    bool active;
     if (c < ex * (a.fny - 2)) {
         j = 1 + c / ex;
         i = 1 + a.tiles_x * kTX + (c - (j - 1) * ex);
-        active = true;
     } else {
         const int c2 = c - ex * (a.fny - 2);
-        const int w = a.tiles_x * kTX;  // leftover rows at the +y end, tiled columns only
-        j = 1 + a.tiles_y * kTY + (w > 0 ? c2 / w : 0);
-        i = 1 + (w > 0 ? c2 - (j - 1 - a.tiles_y * kTY) * w : 0);
-        active = w > 0 && j <= a.fny - 2;
+        const int w = a.tiles_x * kTX;  // > 0 whenever such columns exist
+        j = 1 + a.tiles_y * kTY + c2 / w;
+        i = 1 + (c2 - (j - 1 - a.tiles_y * kTY) * w);
     }
-    if (!active) return;
-    const int n = k1 - k0;
-    const uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+    const bool live = k < k1;
     const int tile = act_tile_x(i) + act_tile_y(j) * a.ntx;
-    uint64_t need = full;
-    if (a.use_act) {
+    bool need = live;
+    if (live && a.use_act) {
         const int xl = act_tile_x(bc_map(i - 1, a.fnx, a.bcx)), xc = act_tile_x(i);
         const int xr = act_tile_x(bc_map(i + 1, a.fnx, a.bcx));
-        const int yl = act_tile_y(bc_map(j - 1, a.fny, a.bcy)), yc = act_tile_y(j);
-        const int yr = act_tile_y(bc_map(j + 1, a.fny, a.bcy));
-        uint64_t act = 0ull, dst = 0ull;
-        for (int r = 0; r < n + 2; ++r) {
-            const unsigned* w = a.act_src + bc_map(k0 - 1 + r, a.fnz, a.bcz) * a.ntiles;
-            bool on = false;
+        const int rl = act_tile_y(bc_map(j - 1, a.fny, a.bcy)) * a.ntx, rc = act_tile_y(j) * a.ntx;
+        const int rr = act_tile_y(bc_map(j + 1, a.fny, a.bcy)) * a.ntx;
+        const unsigned e = a.ep_src;
+        need = a.act_dst[k * a.ntiles + tile] >= a.ep_dst;
 #pragma unroll
-            for (int qy = 0; qy < 3; ++qy) {
-                const int ty = (qy == 0 ? yl : (qy == 1 ? yc : yr)) * a.ntx;
-                on = on || w[xl + ty] >= a.ep_src || w[xc + ty] >= a.ep_src ||
-                     w[xr + ty] >= a.ep_src;
-            }
-            if (on) act |= 1ull << r;
-            if (r < n && a.act_dst[(k0 + r) * a.ntiles + tile] >= a.ep_dst) dst |= 1ull << r;
+        for (int dz = -1; dz <= 1; ++dz) {
+            const unsigned* w = a.act_src + bc_map(k + dz, a.fnz, a.bcz) * a.ntiles;
+            need = need || w[xl + rl] >= e || w[xc + rl] >= e || w[xr + rl] >= e ||
+                   w[xl + rc] >= e || w[xc + rc] >= e || w[xr + rc] >= e ||
+                   w[xl + rr] >= e || w[xc + rr] >= e || w[xr + rr] >= e;
         }
-        need = ((act | (act >> 1) | (act >> 2)) | dst) & full;
     }
-    for (int k = k0; k < k1; ++k) {
-        if (!((need >> (k - k0)) & 1ull)) continue;
-        const int r = apt_point<K, G27>(a, i, j, k, a_s, epsv);
-        if (r != 0) a.act_dst[k * a.ntiles + tile] = a.ep_new;
-        fixup_append(a, r == 2, (unsigned)(i + j * a.fnx + k * A));
+    int res = 0;
+    if (need) {
+        res = apt_point<K, G27>(a, i, j, k, a_s, epsv);
+        if (res != 0) a.act_dst[k * a.ntiles + tile] = a.ep_new;
+    }
+    fixup_append(a, res == 2, (unsigned)(i + j * a.fnx + k * a.fnx * a.fny));
+}
+
+// Work list of one step (runs before apt_step_tiled): one lane per (z chunk,
+// full tile).  A tile chunk is listed only if some plane of it needs
+// processing, with its plane masks.  Edge items are not listed: the first
+// nzchunk * edge_items work indices are edge items, always taken (their
+// points decide one by one).
+__global__ __launch_bounds__(kBlock) void apt_plan(StepArgs a)
+{
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    // the counter set of the next step (last used by the previous one, whose
+    // fix-up has finished) starts from zero
+    if (g < 4) a.cnt_next[g] = 0u;
+    const bool valid = g < a.nzchunk * a.items_chunk;
+    uint64_t need = 0ull, act = 0ull;
+    if (valid) {
+        const int zc = g / a.items_chunk;
+        const int t = g - zc * a.items_chunk;
+        const int ty = t / a.tiles_x;
+        const int tx = t - ty * a.tiles_x;
+        const int k0 = 1 + zc * a.kchunk;
+        const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+        chunk_masks(a, tx, ty, k0, k1, need, act);
+    }
+    const bool put = valid && need != 0ull;
+    const unsigned long long m = __ballot(put);
+    if (m == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(a.wl_count, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (put) {
+        const unsigned q = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+        a.wl_item[q] = g;
+        a.wl_need[q] = need;
+        a.wl_act[q] = act;
     }
 }
 
-// APTrhs_psi (3DgrainMPI/APTPhaseField.cu:178-354) over every interior point.
-// The hardware hands workgroup b to XCD b % 8.  Each XCD gets, for every z
-// chunk in turn, an equal share of that chunk's work: first its edge-column
-// blocks (edge block e goes to XCD e % 8), then a contiguous run of tiles
-// (x fastest), so y-neighbour tiles share halo rows in one L2.  With plane
-// skipping the active planes sit in a few z chunks, and this spreads them
-// over the whole chip instead of the few XCDs a plain linear order would use.
+// APTrhs_psi (3DgrainMPI/APTPhaseField.cu:178-354) over every interior point:
+// a persistent grid of step_blocks workgroups takes work items one at a time,
+// the edge items first (direct path, slowest per point), then the listed tile
+// chunks.  Every workgroup leaves once the items are exhausted; nothing waits
+// on another workgroup.
 template <int K, bool G27>
 __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
 {
-    const int x = blockIdx.x & 7, s = blockIdx.x >> 3;
-    const int ebx = (a.edge_blocks + 7) >> 3;             // edge slots per XCD per chunk
-    const int ntile = a.tiles_x * a.tiles_y;
-    const int tpx = (ntile + 7) >> 3;                     // tile slots per XCD per chunk
-    const int per = ebx + tpx;
-    const int zc = s / per, r = s - zc * per;
-    if (zc >= a.nzchunk) return;
+    __shared__ unsigned s_q;
     const float a_s = 1.0f + 3.0f * a.kin_delta;
     const float epsv = -4.0f * a.kin_delta / a_s;
-    const int k0 = 1 + zc * a.kchunk;
-    const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-    if (r < ebx) {
-        const int e = r * 8 + x;
-        if (e < a.edge_blocks) edge_march<K, G27>(a, e * kBlock + threadIdx.x, k0, k1, a_s, epsv);
-    } else {
-        const int t = x * tpx + (r - ebx);
-        if (t >= ntile) return;
-        const int ty = t / a.tiles_x;
-        const int tx = t - ty * a.tiles_x;
-        tile_march<K, G27>(a, tx, ty, k0, k1, a_s, epsv);
+    const unsigned ne = (unsigned)(a.nzchunk * a.edge_items);
+    const unsigned n = ne + __atomic_load_n(a.wl_count, __ATOMIC_RELAXED);
+    for (;;) {
+        // the next item is taken only when this workgroup is free, which keeps
+        // the tail of the list balanced
+        if (threadIdx.x == 0) s_q = atomicAdd(a.wl_count + 1, 1u);
+        __syncthreads();
+        const unsigned q = s_q;
+        __syncthreads();
+        if (q >= n) break;
+        if (q < ne) {
+            const int zc = (int)q / a.edge_items;
+            const int e = (int)q - zc * a.edge_items;
+            const int k0 = 1 + zc * a.kchunk;
+            const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+            edge_point<K, G27>(a, e * kBlock + threadIdx.x, k0, k1, a_s, epsv);
+        } else {
+            const unsigned w = q - ne;
+            const int g = a.wl_item[w];
+            const int zc = g / a.items_chunk;
+            const int t = g - zc * a.items_chunk;
+            const int ty = t / a.tiles_x;
+            const int tx = t - ty * a.tiles_x;
+            const int k0 = 1 + zc * a.kchunk;
+            const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+            tile_march<K, G27>(a, tx, ty, k0, k1, a.wl_need[w], a.wl_act[w], a_s, epsv);
+        }
     }
 }
 
 // Fix-up pass for the points of the last apt_step launch whose neighbourhood
-// overflowed K slots.  The last workgroup to finish adds the count to the
-// overflow diagnostic and rearms the list for the next step.
+// overflowed K slots; adds their number to the overflow diagnostic.  The list
+// lives in this step's counter set, which the next-but-one plan kernel
+// clears, so nothing here has to wait for the other workgroups.
 template <int K, bool G27>
 __global__ __launch_bounds__(kBlock) void apt_fixup(StepArgs a)
 {
     const unsigned n = __atomic_load_n(a.fix_count, __ATOMIC_RELAXED);
+    if (n == 0u) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.overflow, (unsigned long long)n);
     const float a_s = 1.0f + 3.0f * a.kin_delta;
     const float epsv = -4.0f * a.kin_delta / a_s;
     const int A = a.fnx * a.fny;
@@ -1016,15 +1098,6 @@ __global__ __launch_bounds__(kBlock) void apt_fixup(StepArgs a)
         const int j = r / a.fnx;
         const int i = r - j * a.fnx;
         apt_point_overflow<K, G27>(a, i, j, k, a_s, epsv);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(a.fix_count + 1, 1u) == gridDim.x - 1) {
-            atomicAdd(a.overflow, (unsigned long long)n);
-            atomicExch(a.fix_count, 0u);
-            atomicExch(a.fix_count + 1, 0u);
-        }
     }
 }
 
@@ -1327,11 +1400,12 @@ inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,
                        hipEvent_t ev_begin, hipEvent_t ev_mid, hipEvent_t ev_end)
 {
-    const int nblocks = 8 * ((a.edge_blocks + 7) / 8 + (a.tiles_x * a.tiles_y + 7) / 8) * a.nzchunk;
-    dim3 grid(nblocks), block(kBlock), fgrid(kFixupBlocks);
+    dim3 grid(a.step_blocks), block(kBlock), fgrid(kFixupBlocks);
+    dim3 pgrid(max(1, cdiv((long long)a.nzchunk * a.items_chunk, kBlock)));
 #define CUPF_LAUNCH(KK, GG)                                                                   \
     do {                                                                                      \
         if (ev_begin) (void)hipEventRecord(ev_begin, stream);                                 \
+        hipLaunchKernelGGL(apt_plan, pgrid, block, 0, stream, a);                             \
         hipLaunchKernelGGL((apt_step_tiled<KK, GG>), grid, block, 0, stream, a);              \
         if (ev_mid) (void)hipEventRecord(ev_mid, stream);                                     \
         hipLaunchKernelGGL((apt_fixup<KK, GG>), fgrid, block, 0, stream, a);                  \
@@ -1348,6 +1422,24 @@ hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t strea
     }
 #undef CUPF_LAUNCH
     return hipGetLastError();
+}
+
+int step_kernel_slots(int K, int gather27, int device)
+{
+    int ncu = 0, per = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (!gather27 && K == 5)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<5, false>, kBlock, 0);
+    else if (!gather27 && K == 8)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<8, false>, kBlock, 0);
+    else if (gather27 && K == 5)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<5, true>, kBlock, 0);
+    else if (gather27 && K == 8)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<8, true>, kBlock, 0);
+    if (e != hipSuccess || per < 1) return 0;
+    return per * ncu;
 }
 
 hipError_t launch_act_rebuild(const float* ph, const int* arg, unsigned* act, unsigned ep, int fnx,

@@ -36,7 +36,7 @@ struct pf_ctx {
     int* alpha = nullptr;           // collect target / initial alpha (alpha_m)
     unsigned long long* counters = nullptr;  // [0] overflow points, [1] nucleation events
     unsigned* fix_list = nullptr;   // overflow fix-up list, one entry per interior point
-    unsigned* fix_count = nullptr;  // [0] entries, [1] finished fix-up workgroups
+    unsigned* fix_count = nullptr;  // two 4-word counter sets, used by alternate steps
     int* sink = nullptr;            // 128 words written by lanes that must not store
     long long* sums = nullptr;
     std::vector<long long> h_sums;
@@ -47,7 +47,7 @@ struct pf_ctx {
     float* ball = nullptr;
     int cnx = 0, cny = 0, cnz = 0;
     unsigned long long ncall = 0;
-    int kchunk = 8;                 // z planes per workgroup march
+    int kchunk = 4;                 // z planes per work item
     // Activity map per buffer (one word per 64x4 column tile and z plane) and
     // the epoch each buffer was last written at; see StepArgs.
     unsigned* act[2] = {nullptr, nullptr};
@@ -55,6 +55,13 @@ struct pf_ctx {
     unsigned ep_ctr = 1u;
     int use_act = 1;
     int ntx = 0, nty = 0, ntiles = 0;
+    // Work list of the step (filled by the plan kernel, cleared by the fix-up).
+    int* wl_item = nullptr;
+    unsigned long long* wl_need = nullptr;
+    unsigned long long* wl_act = nullptr;
+    unsigned long long nlaunch = 0;  // step launches so far (selects the counter set)
+    int nzchunk = 0, edge_cols = 0, edge_items = 0, items_chunk = 0;
+    int step_blocks = 0;            // persistent step workgroups (occupancy x CUs)
     bool have_coords = false, have_orient = false, have_state = false;
 };
 
@@ -113,7 +120,12 @@ StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
     a.psi_mac = c->psi_mac;
     a.overflow = c->counters;
     a.fix_list = c->fix_list;
-    a.fix_count = c->fix_count;
+    {
+        const int set = (int)(c->nlaunch & 1ull);  // advanced once the launch succeeded
+        a.fix_count = c->fix_count + 4 * set;
+        a.cnt_next = c->fix_count + 4 * (1 - set);
+        a.wl_count = a.fix_count + 2;
+    }
     a.sink_i = c->sink;
     a.sink_f = reinterpret_cast<float*>(c->sink + 64);
     a.fnx = p.fnx;
@@ -123,7 +135,7 @@ StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
     a.ncolx = p.fnx - 2;
     a.ncols = (p.fnx - 2) * (p.fny - 2);
     a.kchunk = c->kchunk;
-    a.nzchunk = (p.fnz - 2 + a.kchunk - 1) / a.kchunk;
+    a.nzchunk = c->nzchunk;
     a.tiles_x = (p.fnx - 2) / 64;
     a.tiles_y = (p.fny - 2) / 4;
     a.act_src = c->act[src];
@@ -135,12 +147,14 @@ StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
     a.ntx = c->ntx;
     a.nty = c->nty;
     a.ntiles = c->ntiles;
-    {
-        const int ex = (p.fnx - 2) - a.tiles_x * 64;
-        const int ey = (p.fny - 2) - a.tiles_y * 4;
-        const long long edge_cols = (long long)ex * (p.fny - 2) + (long long)ey * a.tiles_x * 64;
-        a.edge_blocks = (int)((edge_cols + 255) / 256);
-    }
+    a.edge_cols = c->edge_cols;
+    a.edge_items = c->edge_items;
+    a.items_chunk = c->items_chunk;
+    a.step_blocks = c->step_blocks;
+    a.wl_item = c->wl_item;
+    a.wl_need = c->wl_need;
+    a.wl_act = c->wl_act;
+
     a.bcx = p.bcx;
     a.bcy = p.bcy;
     a.bcz = p.bcz;
@@ -260,7 +274,8 @@ void free_ctx(pf_ctx* c)
     void* bufs[] = {c->ph[0], c->ph[1], c->arg[0], c->arg[1], c->x, c->y, c->z, c->zbuf,
                     c->R12, c->psi_mac, c->T3d, c->phi0, c->phi, c->alpha, c->counters,
                     c->sums, c->alpha_full, c->loss, c->nstatus, c->ball,
-                    c->fix_list, c->fix_count, c->sink, c->act[0], c->act[1]};
+                    c->fix_list, c->fix_count, c->sink, c->act[0], c->act[1],
+                    c->wl_item, c->wl_need, c->wl_act};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -309,6 +324,22 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
     c->ntx = (p.fnx - 2 + 63) / 64;
     c->nty = (p.fny - 2 + 3) / 4;
     c->ntiles = c->ntx * c->nty;
+    {
+        // full 64x4 tiles take the LDS path; the leftover columns (leftover x
+        // columns over all rows, leftover y rows under the full tiles) go to
+        // edge items of 256 points each
+        const int tiles_x = (p.fnx - 2) / 64, tiles_y = (p.fny - 2) / 4;
+        const int ex = (p.fnx - 2) - tiles_x * 64, ey = (p.fny - 2) - tiles_y * 4;
+        c->edge_cols = ex * (p.fny - 2) + ey * tiles_x * 64;
+        c->edge_items = (int)(((long long)c->kchunk * c->edge_cols + 255) / 256);
+        c->items_chunk = tiles_x * tiles_y;
+        c->nzchunk = (p.fnz - 2 + c->kchunk - 1) / c->kchunk;
+    }
+    c->step_blocks = step_kernel_slots(p.K, p.gather27, device);
+    if (c->step_blocks < 1) {
+        delete c;
+        return fail(PF_E_HIP, "occupancy query for the step kernel failed");
+    }
     c->device = device;
     c->L = (int)L;
     int rc = PF_OK;
@@ -329,9 +360,12 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         (rc = dalloc(&c->phi, (size_t)L)) || (rc = dalloc(&c->alpha, (size_t)L)) ||
         (rc = dalloc(&c->counters, 2)) || (rc = dalloc(&c->sums, p.fnz)) ||
         (rc = dalloc(&c->fix_list, (size_t)(p.fnx - 2) * (p.fny - 2) * (p.fnz - 2))) ||
-        (rc = dalloc(&c->fix_count, 2)) || (rc = dalloc(&c->sink, 128)) ||
+        (rc = dalloc(&c->fix_count, 8)) || (rc = dalloc(&c->sink, 128)) ||
         (rc = dalloc(&c->act[0], (size_t)c->ntiles * p.fnz)) ||
-        (rc = dalloc(&c->act[1], (size_t)c->ntiles * p.fnz)))
+        (rc = dalloc(&c->act[1], (size_t)c->ntiles * p.fnz)) ||
+        (rc = dalloc(&c->wl_item, (size_t)c->nzchunk * c->items_chunk + 1)) ||
+        (rc = dalloc(&c->wl_need, (size_t)c->nzchunk * c->items_chunk + 1)) ||
+        (rc = dalloc(&c->wl_act, (size_t)c->nzchunk * c->items_chunk + 1)))
         return bail(rc);
     c->h_sums.resize(p.fnz);
     for (int b = 0; b < 2; ++b) {
@@ -340,7 +374,7 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
             return bail(fail(PF_E_HIP, "initial fill failed"));
     }
     if (hipMemsetAsync(c->counters, 0, 2 * sizeof(unsigned long long), c->stream) != hipSuccess ||
-        hipMemsetAsync(c->fix_count, 0, 2 * sizeof(unsigned), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->fix_count, 0, 8 * sizeof(unsigned), c->stream) != hipSuccess ||
         hipMemsetAsync(c->alpha, 0, L * sizeof(int), c->stream) != hipSuccess ||
         hipMemsetAsync(c->act[0], 0, (size_t)c->ntiles * p.fnz * sizeof(unsigned), c->stream) != hipSuccess ||
         hipMemsetAsync(c->act[1], 0, (size_t)c->ntiles * p.fnz * sizeof(unsigned), c->stream) != hipSuccess ||
@@ -528,6 +562,7 @@ static int step_impl(pf_ctx* c, int nsteps, int first_step, int flags)
         HIPCHK(act_renew_if_needed(c));
         StepArgs a = make_step_args(c, c->cur, t, use_iuc);
         HIPCHK(launch_step(a, p.K, p.gather27, c->stream));
+        ++c->nlaunch;
         c->ep[1 - c->cur] = a.ep_new;
         c->cur = 1 - c->cur;
         if (nucl && (s & 1)) {
@@ -580,6 +615,8 @@ int pf_step_profiled(pf_ctx* c, int nsteps, int first_step, int flags, float* ms
         if (err != hipSuccess) break;
         StepArgs a = make_step_args(c, c->cur, t, use_iuc);
         err = launch_step(a, p.K, p.gather27, c->stream, ev[3 * q], ev[3 * q + 1], ev[3 * q + 2]);
+        if (err != hipSuccess) break;
+        ++c->nlaunch;
         c->ep[1 - c->cur] = a.ep_new;
         c->cur = 1 - c->cur;
     }
