@@ -13,6 +13,11 @@ constexpr double kLS = -0.995;
 // Slot kill threshold (3DgrainMPI/APTPhaseField.cu:341), also a double compare.
 constexpr double kKill = -0.9999;
 
+// Leftover columns at the +x end narrower than this take the direct path;
+// wider ones form a partial LDS tile (its idle lanes cost less than the direct
+// path's per-point neighbour loads).
+constexpr int kPartialMin = 16;
+
 // Everything one APT step launch reads.  Passed by value (kernel argument
 // segment -> SGPRs), so no constant-memory upload per step.
 struct StepArgs {
@@ -62,7 +67,8 @@ struct StepArgs {
     int L;                 // slot stride fnx*fny*fnz
     int ncolx, ncols;      // interior columns: (fnx-2), (fnx-2)*(fny-2)
     int nzchunk, kchunk;   // z-chunks per column, planes per chunk
-    int tiles_x, tiles_y;  // full 64x4 column tiles (LDS path)
+    int tiles_x, tiles_y;  // 64x4 column tiles of the LDS path (partial ones at the +x/+y
+                           // end included; see kPartialMin)
     int bcx, bcy, bcz;
     int num_theta;
     int thermal_type;

@@ -689,6 +689,9 @@ struct PlaneLoader {
     bool live;                                            // staged plane holds non-empty slots
 };
 
+// Staging offsets of a tile.  A partial tile (at the +x or +y end) clamps the
+// positions past the halo to the halo column/row; only its inactive lanes
+// read those entries.
 template <int K>
 __device__ __forceinline__ void loader_init(PlaneLoader<K>& ld, const StepArgs& a, int i0, int j0)
 {
@@ -700,8 +703,8 @@ __device__ __forceinline__ void loader_init(PlaneLoader<K>& ld, const StepArgs& 
         const int ly = pt / kHX;
         const int lx = pt - ly * kHX;
         ld.off[q] = (e < PlaneLoader<K>::NE)
-                        ? bc_map(i0 - 1 + lx, a.fnx, a.bcx) + bc_map(j0 - 1 + ly, a.fny, a.bcy) * a.fnx
-                              + s * a.L
+                        ? bc_map(min(i0 - 1 + lx, a.fnx - 1), a.fnx, a.bcx)
+                              + bc_map(min(j0 - 1 + ly, a.fny - 1), a.fny, a.bcy) * a.fnx + s * a.L
                         : -1;
     }
 }
@@ -816,8 +819,9 @@ __device__ __forceinline__ void fixup_append(const StepArgs& a, int ovf, unsigne
 template <int K, bool G27>
 __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld, int (*rid)[K * kNPT],
                                            float (*rph)[K * kNPT], const float* zs, int i, int j,
-                                           int k, int k0, int tile, int pt, float xi, float yj,
-                                           float a_s, float epsv, bool stage_next, bool act_next2)
+                                           int k, int k0, int tile, int pt, bool lane_on, float xi,
+                                           float yj, float a_s, float epsv, bool stage_next,
+                                           bool act_next2)
 {
     const int A = a.fnx * a.fny;
     __syncthreads();
@@ -825,6 +829,10 @@ __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld
     Gather<K> g;
     gather_init(g);
     scan_all_lds<K, G27>(rid[sD], rid[sC], rid[sU], pt, g);
+    if (!lane_on) {  // lane past the end of a partial tile: nothing to compute or store
+        g.n = 0;
+        g.ovf = false;
+    }
     const int ovf = g.ovf ? 1 : 0;
     const int C = i + j * a.fnx + k * A;
     const float* pD = rph[sD];
@@ -855,7 +863,7 @@ __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-        const bool w = !ovf && (wr || q >= g.n);
+        const bool w = lane_on && !ovf && (wr || q >= g.n);
         int* pa = w ? a.arg_new + C + q * a.L : a.sink_i + lane;
         float* pp = w ? a.ph_new + C + q * a.L : a.sink_f + lane;
         *pa = oid[q];
@@ -886,8 +894,11 @@ __device__ __forceinline__ void chunk_masks(const StepArgs& a, int tx, int ty, i
         return;
     }
     const int i0 = 1 + tx * kTX, j0 = 1 + ty * kTY;
-    const int xl = act_tile_x(bc_map(i0 - 1, a.fnx, a.bcx)), xr = act_tile_x(bc_map(i0 + kTX, a.fnx, a.bcx));
-    const int yl = act_tile_y(bc_map(j0 - 1, a.fny, a.bcy)), yr = act_tile_y(bc_map(j0 + kTY, a.fny, a.bcy));
+    // halo columns/rows of the tile (a partial tile's end at the last interior point)
+    const int xl = act_tile_x(bc_map(i0 - 1, a.fnx, a.bcx));
+    const int xr = act_tile_x(bc_map(min(i0 + kTX, a.fnx - 1), a.fnx, a.bcx));
+    const int yl = act_tile_y(bc_map(j0 - 1, a.fny, a.bcy));
+    const int yr = act_tile_y(bc_map(min(j0 + kTY, a.fny - 1), a.fny, a.bcy));
     const int rl = yl * a.ntx, rc = ty * a.ntx, rr = yr * a.ntx;
     uint64_t ac = 0ull, dst = 0ull;
     for (int r = 0; r < n + 2; ++r) {
@@ -921,7 +932,8 @@ __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, in
     auto need_at = [&](int kk) { return kk < k1 && ((need >> (kk - k0)) & 1ull) != 0ull; };
     // coordinates for the thermal field: per lane once, z through LDS (made
     // visible by the barrier that opens every run)
-    const float xi = a.x[i], yj = a.y[j];
+    const bool lane_on = i <= a.fnx - 2 && j <= a.fny - 2;
+    const float xi = a.x[min(i, a.fnx - 2)], yj = a.y[min(j, a.fny - 2)];
     if ((int)threadIdx.x < k1 - k0) zs[threadIdx.x] = a.z[k0 + threadIdx.x];
     PlaneLoader<K> ld;
     loader_init(ld, a, i0, j0);
@@ -941,20 +953,20 @@ __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, in
         loader_fetch(ld, a, bc_map(k + 1, a.fnz, a.bcz) * A, act_at(k + 1));
         loader_store(ld, rid[(k + 1) % 3], rph[(k + 1) % 3]);
         do {
-            tile_plane<K, G27>(a, ld, rid, rph, zs, i, j, k, k0, tile, pt, xi, yj, a_s, epsv,
-                               need_at(k + 1), act_at(k + 2));
+            tile_plane<K, G27>(a, ld, rid, rph, zs, i, j, k, k0, tile, pt, lane_on, xi, yj, a_s,
+                               epsv, need_at(k + 1), act_at(k + 2));
             ++k;
         } while (need_at(k));
     }
 }
 
-// Columns outside the full tiles (the strip left when fnx-2 or fny-2 is not
-// a multiple of the tile size) take the direct path, one point per lane:
-// point p of z chunk [k0, k1) is edge column p % edge_cols at plane
-// k0 + p / edge_cols.  Edge columns are numbered leftover x columns first
-// (all rows), then leftover y rows under the full tiles.  A point is skipped
-// under the same rule as a tile plane: all of its stencil's tile planes empty
-// in the source and its own tile plane empty in the destination.
+// A narrow strip of columns left at the +x end (fewer than kPartialMin; wider
+// leftovers, and every leftover in y, are partial LDS tiles) takes the direct
+// path, one point per lane: point p of z chunk [k0, k1) is strip column
+// p % edge_cols (row-major over the strip) at plane k0 + p / edge_cols.  A
+// point is skipped under the same rule as a tile plane: all of its stencil's
+// tile planes empty in the source and its own tile plane empty in the
+// destination.
 template <int K, bool G27>
 __device__ __forceinline__ void edge_point(const StepArgs& a, int p, int k0, int k1, float a_s,
                                            float epsv)
@@ -962,17 +974,9 @@ __device__ __forceinline__ void edge_point(const StepArgs& a, int p, int k0, int
     const int r = p / a.edge_cols;
     const int c = p - r * a.edge_cols;
     const int k = k0 + r;
-    const int ex = (a.fnx - 2) - a.tiles_x * kTX;
-    int i, j;
-    if (c < ex * (a.fny - 2)) {
-        j = 1 + c / ex;
-        i = 1 + a.tiles_x * kTX + (c - (j - 1) * ex);
-    } else {
-        const int c2 = c - ex * (a.fny - 2);
-        const int w = a.tiles_x * kTX;  // > 0 whenever such columns exist
-        j = 1 + a.tiles_y * kTY + c2 / w;
-        i = 1 + (c2 - (j - 1 - a.tiles_y * kTY) * w);
-    }
+    const int ex = (a.fnx - 2) - a.tiles_x * kTX;  // strip width, > 0 whenever edge items exist
+    const int j = 1 + c / ex;
+    const int i = 1 + a.tiles_x * kTX + (c - (j - 1) * ex);
     const bool live = k < k1;
     const int tile = act_tile_x(i) + act_tile_y(j) * a.ntx;
     bool need = live;

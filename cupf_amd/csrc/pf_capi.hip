@@ -61,6 +61,7 @@ struct pf_ctx {
     unsigned long long* wl_act = nullptr;
     unsigned long long nlaunch = 0;  // step launches so far (selects the counter set)
     int nzchunk = 0, edge_cols = 0, edge_items = 0, items_chunk = 0;
+    int tiles_x = 0, tiles_y = 0;   // LDS tiles (partial ones included)
     int step_blocks = 0;            // persistent step workgroups (occupancy x CUs)
     bool have_coords = false, have_orient = false, have_state = false;
 };
@@ -136,8 +137,8 @@ StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
     a.ncols = (p.fnx - 2) * (p.fny - 2);
     a.kchunk = c->kchunk;
     a.nzchunk = c->nzchunk;
-    a.tiles_x = (p.fnx - 2) / 64;
-    a.tiles_y = (p.fny - 2) / 4;
+    a.tiles_x = c->tiles_x;
+    a.tiles_y = c->tiles_y;
     a.act_src = c->act[src];
     a.act_dst = c->act[1 - src];
     a.ep_src = c->ep[src];
@@ -325,14 +326,20 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
     c->nty = (p.fny - 2 + 3) / 4;
     c->ntiles = c->ntx * c->nty;
     {
-        // full 64x4 tiles take the LDS path; the leftover columns (leftover x
-        // columns over all rows, leftover y rows under the full tiles) go to
+        // 64x4 tiles take the LDS path, a partial tile where the +x leftover is
+        // at least kPartialMin wide and always in y; a narrower +x strip goes to
         // edge items of 256 points each
-        const int tiles_x = (p.fnx - 2) / 64, tiles_y = (p.fny - 2) / 4;
-        const int ex = (p.fnx - 2) - tiles_x * 64, ey = (p.fny - 2) - tiles_y * 4;
-        c->edge_cols = ex * (p.fny - 2) + ey * tiles_x * 64;
+        int tiles_x = (p.fnx - 2) / 64;
+        int ex = (p.fnx - 2) - tiles_x * 64;
+        if (ex >= kPartialMin) {
+            ++tiles_x;
+            ex = 0;
+        }
+        c->tiles_x = tiles_x;
+        c->tiles_y = (p.fny - 2 + 3) / 4;
+        c->edge_cols = ex * (p.fny - 2);
         c->edge_items = (int)(((long long)c->kchunk * c->edge_cols + 255) / 256);
-        c->items_chunk = tiles_x * tiles_y;
+        c->items_chunk = c->tiles_x * c->tiles_y;
         c->nzchunk = (p.fnz - 2 + c->kchunk - 1) / c->kchunk;
     }
     c->step_blocks = step_kernel_slots(p.K, p.gather27, device);

@@ -50,6 +50,8 @@ def test_step_small_bitexact(K):
     ((130, 9, 12), 5, 0, (0, 1, 0)),    # tiles plus leftover columns in x and rows in y
     ((63, 12, 40), 8, 0, (1, 0, 1)),    # one tile column, leftover row, two z chunks
     ((130, 9, 12), 8, 1, (1, 1, 0)),    # 27-point gather through the LDS tiles
+    ((95, 10, 20), 8, 0, (1, 1, 0)),    # one full tile and a 32-column partial tile in x
+    ((32, 32, 16), 5, 1, (0, 1, 0)),    # a single 33-column partial tile, 27-point gather
 ])
 def test_tiled_path_shapes(shape, K, g27, bc):
     c = cases.make_line_case("tiles", n=shape, grains=12, K=K, seed=43, z0_cells=shape[2] // 2,
