@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libcupf_amd.so")
 PF_OK = 0
 PF_STEP_INITIAL_UC = 1
 PF_STEP_NUCLEATE = 2
+PF_BATCH_GRAPH = 4
+PF_BATCH_AUTOTUNE = 8
 PF_COLLECT_CLEAR_ALPHA = 1
 PF_COLLECT_NO_BC = 2
 
@@ -111,6 +113,11 @@ def load() -> ctypes.CDLL:
         "pf_overflow_count": ([vp, llp], c_int),
         "pf_sync": ([vp], c_int),
         "pf_activity": ([vp, c_int, ctypes.c_void_p, llp, llp], c_int),
+        "pf_batch_create": ([ctypes.POINTER(vp), c_int, ctypes.POINTER(vp)], c_int),
+        "pf_batch_destroy": ([vp], c_int),
+        "pf_batch_step": ([vp, c_int, ip, c_int], c_int),
+        "pf_batch_set_kchunk": ([vp, c_int], c_int),
+        "pf_batch_get_kchunk": ([vp, ip], c_int),
         "pf_copy_bandwidth": ([c_int, ctypes.c_ulonglong, c_int, fp], c_int),
     }
     for name, (argt, rest) in sig.items():
@@ -288,3 +295,46 @@ class APTContext:
         _check(load().pf_activity(self._h, which, None if m is None else m.ctypes.data,
                                   ctypes.byref(a), ctypes.byref(t)))
         return (a.value, t.value, m) if with_map else (a.value, t.value)
+
+
+class Batch:
+    """Several APTContexts of one device stepped together (pf_batch_*).
+
+    Each member keeps its own step counter here; step(n) advances every member
+    by n steps from its counter.  Members remain usable on their own between
+    calls (download, collect, single steps).
+    """
+
+    def __init__(self, members, first_step=None):
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m._h for m in self.members])
+        h = ctypes.c_void_p()
+        _check(load().pf_batch_create(arr, len(self.members), ctypes.byref(h)))
+        self._h = h
+        self.next_step = list(first_step) if first_step is not None else [1] * len(self.members)
+
+    def step(self, n: int, flags: int = 0) -> None:
+        fs = (ctypes.c_int * len(self.members))(*self.next_step)
+        _check(load().pf_batch_step(self._h, int(n), fs, int(flags)))
+        self.next_step = [s + int(n) for s in self.next_step]
+
+    @property
+    def kchunk(self) -> int:
+        v = ctypes.c_int()
+        _check(load().pf_batch_get_kchunk(self._h, ctypes.byref(v)))
+        return v.value
+
+    @kchunk.setter
+    def kchunk(self, v: int) -> None:
+        _check(load().pf_batch_set_kchunk(self._h, int(v)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load().pf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

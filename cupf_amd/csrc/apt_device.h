@@ -1,4 +1,5 @@
-// apt_kernels.hip -- gfx950 kernels of the APT multi-grain phase-field step.
+// apt_device.h -- device code of the APT step shared by the kernel translation
+// units (included by apt_misc.hip and step_k*.hip; internal linkage).
 //
 // Field layout (3DgrainMPI/devicefunc.cu_inl:24-29): SoA slot planes,
 // element (i,j,k,slot) at i + j*fnx + k*fnx*fny + slot*L, phi float32 and
@@ -7,6 +8,8 @@
 // Arithmetic follows the reference source operation by operation (build with
 // -ffp-contract=off): float everywhere except the reference's implicit double
 // promotions (the 1e6 thermal factors, LS=-0.995 and the -0.9999 kill test).
+#pragma once
+
 #include "apt_internal.h"
 
 namespace cupf {
@@ -230,6 +233,16 @@ __device__ __forceinline__ int nbase(const StepArgs& a, const Nbr& nb, int xi, i
     return nb.xs[xi] + nb.ys[yi] * a.fnx + nb.zs[zi] * a.fnx * a.fny;
 }
 
+// Base of z-major neighbour position q = xi + 3*yi + 9*zi (0..26) chosen at run time.
+__device__ __forceinline__ int nbase_q(const StepArgs& a, const Nbr& nb, int q)
+{
+    const int xi = q % 3, yi = (q / 3) % 3, zi = q / 9;
+    const int x = xi == 0 ? nb.xs[0] : (xi == 1 ? nb.xs[1] : nb.xs[2]);
+    const int y = yi == 0 ? nb.ys[0] : (yi == 1 ? nb.ys[1] : nb.ys[2]);
+    const int z = zi == 0 ? nb.zs[0] : (zi == 1 ? nb.zs[1] : nb.zs[2]);
+    return x + y * a.fnx + z * a.fnx * a.fny;
+}
+
 // Stencil row bases D,B,L,C,R,T,U (stencil = -3..3, 3DgrainMPI/APTPhaseField.cu:211-214).
 __device__ __forceinline__ void stencil_bases(const StepArgs& a, const Nbr& nb, int (&b)[7])
 {
@@ -255,19 +268,27 @@ __device__ __forceinline__ void scan_all(const StepArgs& a, const Nbr& nb, Gathe
         scan_point<K, 5>(a.arg, nbase(a, nb, 1, 2, 1), a.L, g);
         scan_point<K, 6>(a.arg, nbase(a, nb, 1, 1, 2), a.L, g);
     } else {
-        // z-major 27-point order (3Dgrain/APTPhaseField.cu:241-268); the stencil
-        // rows keep phi: D=(1,1,0) B=(1,0,1) L=(0,1,1) C R T U=(1,1,2).
-#define CUPF_P(XI, YI, ZI, S) scan_point<K, S>(a.arg, nbase(a, nb, XI, YI, ZI), a.L, g)
-        CUPF_P(0, 0, 0, -1); CUPF_P(1, 0, 0, -1); CUPF_P(2, 0, 0, -1);
-        CUPF_P(0, 1, 0, -1); CUPF_P(1, 1, 0, 0);  CUPF_P(2, 1, 0, -1);
-        CUPF_P(0, 2, 0, -1); CUPF_P(1, 2, 0, -1); CUPF_P(2, 2, 0, -1);
-        CUPF_P(0, 0, 1, -1); CUPF_P(1, 0, 1, 1);  CUPF_P(2, 0, 1, -1);
-        CUPF_P(0, 1, 1, 2);  CUPF_P(1, 1, 1, 3);  CUPF_P(2, 1, 1, 4);
-        CUPF_P(0, 2, 1, -1); CUPF_P(1, 2, 1, 5);  CUPF_P(2, 2, 1, -1);
-        CUPF_P(0, 0, 2, -1); CUPF_P(1, 0, 2, -1); CUPF_P(2, 0, 2, -1);
-        CUPF_P(0, 1, 2, -1); CUPF_P(1, 1, 2, 6);  CUPF_P(2, 1, 2, -1);
-        CUPF_P(0, 2, 2, -1); CUPF_P(1, 2, 2, -1); CUPF_P(2, 2, 2, -1);
-#undef CUPF_P
+        // z-major 27-point order (3Dgrain/APTPhaseField.cu:241-268), position
+        // q = xi + 3*yi + 9*zi.  The stencil rows keep phi: D=4 B=10 L=12 C=13
+        // R=14 T=16 U=22; the other 20 positions only contribute ids and run as
+        // rolled loops (same order) to keep the code size down.
+        auto run = [&](int from, int to) {
+#pragma unroll 1
+            for (int q = from; q < to; ++q) scan_point<K, -1>(a.arg, nbase_q(a, nb, q), a.L, g);
+        };
+        run(0, 4);
+        scan_point<K, 0>(a.arg, nbase_q(a, nb, 4), a.L, g);
+        run(5, 10);
+        scan_point<K, 1>(a.arg, nbase_q(a, nb, 10), a.L, g);
+        run(11, 12);
+        scan_point<K, 2>(a.arg, nbase_q(a, nb, 12), a.L, g);
+        scan_point<K, 3>(a.arg, nbase_q(a, nb, 13), a.L, g);
+        scan_point<K, 4>(a.arg, nbase_q(a, nb, 14), a.L, g);
+        run(15, 16);
+        scan_point<K, 5>(a.arg, nbase_q(a, nb, 16), a.L, g);
+        run(17, 22);
+        scan_point<K, 6>(a.arg, nbase_q(a, nb, 22), a.L, g);
+        run(23, 27);
     }
 }
 
@@ -535,8 +556,9 @@ __device__ __forceinline__ void topk_push(TopK<K>& c, int id, float v, int pos)
     }
 }
 
-template <int K, int P>
-__device__ __forceinline__ void topk_point(const StepArgs& a, int base, TopK<K>& c)
+// Candidates of one neighbour point; P is its position in the scan order.
+template <int K>
+__device__ __forceinline__ void topk_point(const StepArgs& a, int base, int P, TopK<K>& c)
 {
 #pragma unroll
     for (int p = 0; p < K; ++p) {
@@ -587,13 +609,13 @@ __device__ __forceinline__ void gather_overflow(const StepArgs& a, const Nbr& nb
     }
     gather_init(g);
     if (!G27) {
-        topk_point<K, 0>(a, nbase(a, nb, 1, 1, 0), c);
-        topk_point<K, 1>(a, nbase(a, nb, 1, 0, 1), c);
-        topk_point<K, 2>(a, nbase(a, nb, 0, 1, 1), c);
-        topk_point<K, 3>(a, nbase(a, nb, 1, 1, 1), c);
-        topk_point<K, 4>(a, nbase(a, nb, 2, 1, 1), c);
-        topk_point<K, 5>(a, nbase(a, nb, 1, 2, 1), c);
-        topk_point<K, 6>(a, nbase(a, nb, 1, 1, 2), c);
+        topk_point<K>(a, nbase(a, nb, 1, 1, 0), 0, c);
+        topk_point<K>(a, nbase(a, nb, 1, 0, 1), 1, c);
+        topk_point<K>(a, nbase(a, nb, 0, 1, 1), 2, c);
+        topk_point<K>(a, nbase(a, nb, 1, 1, 1), 3, c);
+        topk_point<K>(a, nbase(a, nb, 2, 1, 1), 4, c);
+        topk_point<K>(a, nbase(a, nb, 1, 2, 1), 5, c);
+        topk_point<K>(a, nbase(a, nb, 1, 1, 2), 6, c);
         scan_point_kept<K, 0>(a.arg, nbase(a, nb, 1, 1, 0), a.L, g, c);
         scan_point_kept<K, 1>(a.arg, nbase(a, nb, 1, 0, 1), a.L, g, c);
         scan_point_kept<K, 2>(a.arg, nbase(a, nb, 0, 1, 1), a.L, g, c);
@@ -602,28 +624,26 @@ __device__ __forceinline__ void gather_overflow(const StepArgs& a, const Nbr& nb
         scan_point_kept<K, 5>(a.arg, nbase(a, nb, 1, 2, 1), a.L, g, c);
         scan_point_kept<K, 6>(a.arg, nbase(a, nb, 1, 1, 2), a.L, g, c);
     } else {
-#define CUPF_T(XI, YI, ZI, P) topk_point<K, P>(a, nbase(a, nb, XI, YI, ZI), c)
-        CUPF_T(0, 0, 0, 0);  CUPF_T(1, 0, 0, 1);  CUPF_T(2, 0, 0, 2);
-        CUPF_T(0, 1, 0, 3);  CUPF_T(1, 1, 0, 4);  CUPF_T(2, 1, 0, 5);
-        CUPF_T(0, 2, 0, 6);  CUPF_T(1, 2, 0, 7);  CUPF_T(2, 2, 0, 8);
-        CUPF_T(0, 0, 1, 9);  CUPF_T(1, 0, 1, 10); CUPF_T(2, 0, 1, 11);
-        CUPF_T(0, 1, 1, 12); CUPF_T(1, 1, 1, 13); CUPF_T(2, 1, 1, 14);
-        CUPF_T(0, 2, 1, 15); CUPF_T(1, 2, 1, 16); CUPF_T(2, 2, 1, 17);
-        CUPF_T(0, 0, 2, 18); CUPF_T(1, 0, 2, 19); CUPF_T(2, 0, 2, 20);
-        CUPF_T(0, 1, 2, 21); CUPF_T(1, 1, 2, 22); CUPF_T(2, 1, 2, 23);
-        CUPF_T(0, 2, 2, 24); CUPF_T(1, 2, 2, 25); CUPF_T(2, 2, 2, 26);
-#undef CUPF_T
-#define CUPF_K(XI, YI, ZI, S) scan_point_kept<K, S>(a.arg, nbase(a, nb, XI, YI, ZI), a.L, g, c)
-        CUPF_K(0, 0, 0, -1); CUPF_K(1, 0, 0, -1); CUPF_K(2, 0, 0, -1);
-        CUPF_K(0, 1, 0, -1); CUPF_K(1, 1, 0, 0);  CUPF_K(2, 1, 0, -1);
-        CUPF_K(0, 2, 0, -1); CUPF_K(1, 2, 0, -1); CUPF_K(2, 2, 0, -1);
-        CUPF_K(0, 0, 1, -1); CUPF_K(1, 0, 1, 1);  CUPF_K(2, 0, 1, -1);
-        CUPF_K(0, 1, 1, 2);  CUPF_K(1, 1, 1, 3);  CUPF_K(2, 1, 1, 4);
-        CUPF_K(0, 2, 1, -1); CUPF_K(1, 2, 1, 5);  CUPF_K(2, 2, 1, -1);
-        CUPF_K(0, 0, 2, -1); CUPF_K(1, 0, 2, -1); CUPF_K(2, 0, 2, -1);
-        CUPF_K(0, 1, 2, -1); CUPF_K(1, 1, 2, 6);  CUPF_K(2, 1, 2, -1);
-        CUPF_K(0, 2, 2, -1); CUPF_K(1, 2, 2, -1); CUPF_K(2, 2, 2, -1);
-#undef CUPF_K
+#pragma unroll 1
+        for (int q = 0; q < 27; ++q) topk_point<K>(a, nbase_q(a, nb, q), q, c);
+        auto run = [&](int from, int to) {
+#pragma unroll 1
+            for (int q = from; q < to; ++q)
+                scan_point_kept<K, -1>(a.arg, nbase_q(a, nb, q), a.L, g, c);
+        };
+        run(0, 4);
+        scan_point_kept<K, 0>(a.arg, nbase_q(a, nb, 4), a.L, g, c);
+        run(5, 10);
+        scan_point_kept<K, 1>(a.arg, nbase_q(a, nb, 10), a.L, g, c);
+        run(11, 12);
+        scan_point_kept<K, 2>(a.arg, nbase_q(a, nb, 12), a.L, g, c);
+        scan_point_kept<K, 3>(a.arg, nbase_q(a, nb, 13), a.L, g, c);
+        scan_point_kept<K, 4>(a.arg, nbase_q(a, nb, 14), a.L, g, c);
+        run(15, 16);
+        scan_point_kept<K, 5>(a.arg, nbase_q(a, nb, 16), a.L, g, c);
+        run(17, 22);
+        scan_point_kept<K, 6>(a.arg, nbase_q(a, nb, 22), a.L, g, c);
+        run(23, 27);
     }
 }
 
@@ -780,17 +800,28 @@ __device__ __forceinline__ void scan_all_lds(const int* pD, const int* pC, const
         scan_lds<K, 5>(pC, pt + kHX, g);
         scan_lds<K, 6>(pU, pt, g);
     } else {
-#define CUPF_L(PL, DX, DY, S) scan_lds<K, S>(PL, pt + (DY) * kHX + (DX), g)
-        CUPF_L(pD, -1, -1, -1); CUPF_L(pD, 0, -1, -1); CUPF_L(pD, 1, -1, -1);
-        CUPF_L(pD, -1, 0, -1);  CUPF_L(pD, 0, 0, 0);   CUPF_L(pD, 1, 0, -1);
-        CUPF_L(pD, -1, 1, -1);  CUPF_L(pD, 0, 1, -1);  CUPF_L(pD, 1, 1, -1);
-        CUPF_L(pC, -1, -1, -1); CUPF_L(pC, 0, -1, 1);  CUPF_L(pC, 1, -1, -1);
-        CUPF_L(pC, -1, 0, 2);   CUPF_L(pC, 0, 0, 3);   CUPF_L(pC, 1, 0, 4);
-        CUPF_L(pC, -1, 1, -1);  CUPF_L(pC, 0, 1, 5);   CUPF_L(pC, 1, 1, -1);
-        CUPF_L(pU, -1, -1, -1); CUPF_L(pU, 0, -1, -1); CUPF_L(pU, 1, -1, -1);
-        CUPF_L(pU, -1, 0, -1);  CUPF_L(pU, 0, 0, 6);   CUPF_L(pU, 1, 0, -1);
-        CUPF_L(pU, -1, 1, -1);  CUPF_L(pU, 0, 1, -1);  CUPF_L(pU, 1, 1, -1);
-#undef CUPF_L
+        // z-major 27-point order as in scan_all; non-stencil positions rolled
+        auto run = [&](int from, int to) {
+#pragma unroll 1
+            for (int q = from; q < to; ++q) {
+                const int zi = q / 9;
+                const int* pl = zi == 0 ? pD : (zi == 1 ? pC : pU);
+                scan_lds<K, -1>(pl, pt + ((q / 3) % 3 - 1) * kHX + (q % 3 - 1), g);
+            }
+        };
+        run(0, 4);
+        scan_lds<K, 0>(pD, pt, g);
+        run(5, 10);
+        scan_lds<K, 1>(pC, pt - kHX, g);
+        run(11, 12);
+        scan_lds<K, 2>(pC, pt - 1, g);
+        scan_lds<K, 3>(pC, pt, g);
+        scan_lds<K, 4>(pC, pt + 1, g);
+        run(15, 16);
+        scan_lds<K, 5>(pC, pt + kHX, g);
+        run(17, 22);
+        scan_lds<K, 6>(pU, pt, g);
+        run(23, 27);
     }
 }
 
@@ -1003,536 +1034,8 @@ __device__ __forceinline__ void edge_point(const StepArgs& a, int p, int k0, int
     fixup_append(a, res == 2, (unsigned)(i + j * a.fnx + k * a.fnx * a.fny));
 }
 
-// Work list of one step (runs before apt_step_tiled): one lane per (z chunk,
-// full tile).  A tile chunk is listed only if some plane of it needs
-// processing, with its plane masks.  Edge items are not listed: the first
-// nzchunk * edge_items work indices are edge items, always taken (their
-// points decide one by one).
-__global__ __launch_bounds__(kBlock) void apt_plan(StepArgs a)
-{
-    const int g = blockIdx.x * kBlock + threadIdx.x;
-    // the counter set of the next step (last used by the previous one, whose
-    // fix-up has finished) starts from zero
-    if (g < 4) a.cnt_next[g] = 0u;
-    const bool valid = g < a.nzchunk * a.items_chunk;
-    uint64_t need = 0ull, act = 0ull;
-    if (valid) {
-        const int zc = g / a.items_chunk;
-        const int t = g - zc * a.items_chunk;
-        const int ty = t / a.tiles_x;
-        const int tx = t - ty * a.tiles_x;
-        const int k0 = 1 + zc * a.kchunk;
-        const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-        chunk_masks(a, tx, ty, k0, k1, need, act);
-    }
-    const bool put = valid && need != 0ull;
-    const unsigned long long m = __ballot(put);
-    if (m == 0ull) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)m) - 1;
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(a.wl_count, (unsigned)__popcll(m));
-    base = __shfl(base, leader);
-    if (put) {
-        const unsigned q = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-        a.wl_item[q] = g;
-        a.wl_need[q] = need;
-        a.wl_act[q] = act;
-    }
-}
-
-// APTrhs_psi (3DgrainMPI/APTPhaseField.cu:178-354) over every interior point:
-// a persistent grid of step_blocks workgroups takes work items one at a time,
-// the edge items first (direct path, slowest per point), then the listed tile
-// chunks.  Every workgroup leaves once the items are exhausted; nothing waits
-// on another workgroup.
-template <int K, bool G27>
-__global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
-{
-    __shared__ unsigned s_q;
-    const float a_s = 1.0f + 3.0f * a.kin_delta;
-    const float epsv = -4.0f * a.kin_delta / a_s;
-    const unsigned ne = (unsigned)(a.nzchunk * a.edge_items);
-    const unsigned n = ne + __atomic_load_n(a.wl_count, __ATOMIC_RELAXED);
-    for (;;) {
-        // the next item is taken only when this workgroup is free, which keeps
-        // the tail of the list balanced
-        if (threadIdx.x == 0) s_q = atomicAdd(a.wl_count + 1, 1u);
-        __syncthreads();
-        const unsigned q = s_q;
-        __syncthreads();
-        if (q >= n) break;
-        if (q < ne) {
-            const int zc = (int)q / a.edge_items;
-            const int e = (int)q - zc * a.edge_items;
-            const int k0 = 1 + zc * a.kchunk;
-            const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-            edge_point<K, G27>(a, e * kBlock + threadIdx.x, k0, k1, a_s, epsv);
-        } else {
-            const unsigned w = q - ne;
-            const int g = a.wl_item[w];
-            const int zc = g / a.items_chunk;
-            const int t = g - zc * a.items_chunk;
-            const int ty = t / a.tiles_x;
-            const int tx = t - ty * a.tiles_x;
-            const int k0 = 1 + zc * a.kchunk;
-            const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-            tile_march<K, G27>(a, tx, ty, k0, k1, a.wl_need[w], a.wl_act[w], a_s, epsv);
-        }
-    }
-}
-
-// Fix-up pass for the points of the last apt_step launch whose neighbourhood
-// overflowed K slots; adds their number to the overflow diagnostic.  The list
-// lives in this step's counter set, which the next-but-one plan kernel
-// clears, so nothing here has to wait for the other workgroups.
-template <int K, bool G27>
-__global__ __launch_bounds__(kBlock) void apt_fixup(StepArgs a)
-{
-    const unsigned n = __atomic_load_n(a.fix_count, __ATOMIC_RELAXED);
-    if (n == 0u) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.overflow, (unsigned long long)n);
-    const float a_s = 1.0f + 3.0f * a.kin_delta;
-    const float epsv = -4.0f * a.kin_delta / a_s;
-    const int A = a.fnx * a.fny;
-    for (unsigned q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
-        const int C = (int)a.fix_list[q];
-        const int k = C / A;
-        const int r = C - k * A;
-        const int j = r / a.fnx;
-        const int i = r - j * a.fnx;
-        apt_point_overflow<K, G27>(a, i, j, k, a_s, epsv);
-    }
-}
-
-// Halo fill (APTsetBC3D, 3DgrainMPI/APTPhaseField.cu:93-174).  Each halo cell
-// is written exactly once, from its per-axis mapped interior point, so the
-// result is deterministic at edges and corners (the reference races there)
-// and equals a z-then-y-then-x sequential face fill.
-__global__ __launch_bounds__(kBlock) void bc_fill(float* ph, int* arg, int fnx, int fny, int fnz,
-                                                  int L, int bcx, int bcy, int bcz, int nhalo)
-{
-    int h = blockIdx.x * kBlock + threadIdx.x;
-    if (h >= nhalo) return;
-    const int s = blockIdx.y;
-    const int A = fnx * fny;
-    int i, j, k;
-    if (h < 2 * A) {
-        k = h < A ? 0 : fnz - 1;
-        const int r = h < A ? h : h - A;
-        j = r / fnx;
-        i = r - j * fnx;
-    } else {
-        h -= 2 * A;
-        const int By = fnx * (fnz - 2);
-        if (h < 2 * By) {
-            j = h < By ? 0 : fny - 1;
-            const int r = h < By ? h : h - By;
-            k = 1 + r / fnx;
-            i = r - (k - 1) * fnx;
-        } else {
-            h -= 2 * By;
-            const int Bx = (fny - 2) * (fnz - 2);
-            i = h < Bx ? 0 : fnx - 1;
-            const int r = h < Bx ? h : h - Bx;
-            k = 1 + r / (fny - 2);
-            j = 1 + (r - (k - 1) * (fny - 2));
-        }
-    }
-    const int src = bc_map(i, fnx, bcx) + bc_map(j, fny, bcy) * fnx + bc_map(k, fnz, bcz) * A + s * L;
-    const int dst = i + j * fnx + k * A + s * L;
-    ph[dst] = ph[src];
-    arg[dst] = arg[src];
-}
-
-// APTcollect_PF (3DgrainMPI/APTPhaseField.cu:55-75): argmax over slots with a
-// strict > from slot 0; written only if the max exceeds LS (double compare).
-__global__ __launch_bounds__(kBlock) void collect_kernel(const float* __restrict__ ph,
-                                                         const int* __restrict__ arg,
-                                                         float* __restrict__ phi,
-                                                         int* __restrict__ alpha, int L, int K,
-                                                         int clear_alpha)
-{
-    const int C = blockIdx.x * kBlock + threadIdx.x;
-    if (C >= L) return;
-    int am = 0;
-    float mv = ph[C];
-    for (int s = 1; s < K; ++s) {
-        const float v = ph[C + s * L];
-        if (v > mv) {
-            mv = v;
-            am = s;
-        }
-    }
-    if ((double)mv > kLS) {
-        phi[C] = mv;
-        alpha[C] = arg[C + am * L];
-    } else if (clear_alpha) {
-        alpha[C] = 0;
-    }
-}
-
-// set_minus1 + memset(-1) + APTini_PF on both buffers
-// (3DgrainMPI/APTPhaseField.cu:78-89,727-728,817-818).
-__global__ __launch_bounds__(kBlock) void ini_kernel(float* ph0, int* arg0, float* ph1, int* arg1,
-                                                     const float* __restrict__ phi0,
-                                                     const int* __restrict__ alpha0, int L, int K)
-{
-    const int q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= L * K) return;
-    float v = -1.0f;
-    int id = -1;
-    if (q < L && (double)phi0[q] > kLS) {
-        v = phi0[q];
-        id = alpha0[q];
-    }
-    ph0[q] = v;
-    arg0[q] = id;
-    ph1[q] = v;
-    arg1[q] = id;
-}
-
-// ave_x (3DgrainMPI/PhaseField.cu:366-378) as an exact fixed-point sum:
-// every phi is rounded to a multiple of 2^-40 and summed in int64, which is
-// independent of summation order (the reference's float atomics are not).
-__global__ __launch_bounds__(kBlock) void plane_sum_kernel(const float* __restrict__ ph,
-                                                           long long* sums, int A, int L, int K)
-{
-    __shared__ long long red[kBlock];
-    const int k = blockIdx.y;
-    long long acc = 0;
-    for (int s = 0; s < K; ++s)
-        for (int q = blockIdx.x * kBlock + threadIdx.x; q < A; q += gridDim.x * kBlock)
-            acc += (long long)rint((double)ph[s * L + k * A + q] * 1099511627776.0);
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd((unsigned long long*)&sums[k], (unsigned long long)red[0]);
-}
-
-// APTmove_frame, first half (3DgrainMPI/APTPhaseField.cu:357-391): shift every
-// slot of the interior down one plane into the other buffer (top plane
-// becomes -1/-1), shifted z into zbuf, archive alpha plane 1, count loss area.
-__global__ __launch_bounds__(kBlock) void move_frame_kernel(
-    const float* __restrict__ ph, const int* __restrict__ arg, float* __restrict__ phb,
-    int* __restrict__ argb, const float* __restrict__ z, float* __restrict__ zb,
-    const int* __restrict__ alpha, int* __restrict__ alpha_full, int* __restrict__ loss,
-    int move_count, int fnx, int fny, int fnz, int L, int K, int num_theta)
-{
-    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (q >= (long long)L * K) return;
-    const int s = (int)(q / L);
-    const int C = (int)(q - (long long)s * L);
-    const int A = fnx * fny;
-    const int k = C / A;
-    const int r = C - k * A;
-    const int j = r / fnx;
-    const int i = r - j * fnx;
-    if (i == 0 && j == 0 && s == 0) {
-        if (k > 0 && k < fnz - 2) zb[k] = z[k + 1];
-        if (k == fnz - 2) zb[k] = 2 * z[fnz - 2] - z[fnz - 3];
-    }
-    if (i > 0 && i < fnx - 1 && j > 0 && j < fny - 1) {
-        if (k > 0 && k < fnz - 2) {
-            phb[q] = ph[q + A];
-            argb[q] = arg[q + A];
-        } else if (k == fnz - 2) {
-            phb[q] = -1.0f;
-            argb[q] = -1;
-        }
-    }
-    if (k == 1 && s == 0) {
-        const int al = alpha[C];
-        alpha_full[(long long)move_count * A + C] = al;
-        if (al >= 1 && al <= num_theta) atomicAdd(loss + al - 1, 1);
-    }
-}
-
-// APTcopy_frame (3DgrainMPI/APTPhaseField.cu:393-410).
-__global__ __launch_bounds__(kBlock) void copy_frame_kernel(
-    const float* __restrict__ phb, const int* __restrict__ argb, const float* __restrict__ zb,
-    float* __restrict__ ph, int* __restrict__ arg, float* __restrict__ z, int fnx, int fny,
-    int fnz, int L, int K)
-{
-    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (q >= (long long)L * K) return;
-    const int s = (int)(q / L);
-    const int C = (int)(q - (long long)s * L);
-    const int A = fnx * fny;
-    const int k = C / A;
-    const int r = C - k * A;
-    const int j = r / fnx;
-    const int i = r - j * fnx;
-    if (i == 0 && j == 0 && s == 0 && k > 0 && k < fnz - 1) z[k] = zb[k];
-    if (i > 0 && i < fnx - 1 && j > 0 && j < fny - 1 && k > 0 && k < fnz - 1) {
-        ph[q] = phb[q];
-        arg[q] = argb[q];
-    }
-}
-
-// init_nucl_status (3DgrainMPI/APTPhaseField.cu:420-444), block centre taken
-// without the halo offset exactly as the reference does.
-__global__ __launch_bounds__(kBlock) void nucl_status_kernel(const float* __restrict__ phi0,
-                                                             int* status, int cnx, int cny,
-                                                             int cnz, int p, int fnx, int fny)
-{
-    const int C = blockIdx.x * kBlock + threadIdx.x;
-    if (C >= cnx * cny * cnz) return;
-    const int k = C / (cnx * cny);
-    const int r = C - k * cnx * cny;
-    const int j = r / cnx;
-    const int i = r - j * cnx;
-    const int w = 2 * p + 1;
-    const int g = (w * k + p) * fnx * fny + (w * j + p) * fnx + (w * i + p);
-    status[C] = ((double)phi0[g] > kLS) ? 1 : 0;
-}
-
-// splitmix64 finaliser (Steele, Lea & Flood, OOPSLA 2014).
-__device__ __forceinline__ uint64_t mix64(uint64_t v)
-{
-    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ULL;
-    v = (v ^ (v >> 27)) * 0x94d049bb133111ebULL;
-    return v ^ (v >> 31);
-}
-
-// nuncl_possibility (3DgrainMPI/APTPhaseField.cu:446-455).
-__device__ __forceinline__ float nucl_prob(const NuclArgs& a, float delT, float d_delT, float nmax)
-{
-    float slope = -0.5f * (delT - a.undcool_mean) * (delT - a.undcool_mean) / a.undcool_std
-                / a.undcool_std;
-    slope = expf(slope);
-    float density = nmax / (sqrtf((float)(2.0f * M_PI)) * a.undcool_std) * slope * d_delT;
-    return 8.0f * a.nuc_rad * a.nuc_rad * a.nuc_rad * density;
-}
-
-// add_nucl (3DgrainMPI/APTPhaseField.cu:459-536) with a counter-based draw per
-// (seed, coarse cell, call) instead of per-cell cuRAND state.
-__global__ __launch_bounds__(kBlock) void nucl_kernel(NuclArgs a)
-{
-    const int C = blockIdx.x * kBlock + threadIdx.x;
-    if (C >= a.cnx * a.cny * a.cnz) return;
-    if (a.status[C] != 0) return;
-    const int k = C / (a.cnx * a.cny);
-    const int r = C - k * a.cnx * a.cny;
-    const int j = r / a.cnx;
-    const int i = r - j * a.cnx;
-    const int w = 2 * a.p + 1;
-    const int gi = w * i + a.p + 1, gj = w * j + a.p + 1, gk = w * k + a.p + 1;
-    const int A = a.fnx * a.fny;
-    const int gC = gk * A + gj * a.fnx + gi;
-    int st = 0;
-    for (int s = 0; s < a.K; ++s)
-        if ((double)a.ph[gC + s * a.L] > kLS) st = 1;
-    if (st) {
-        a.status[C] = 1;
-        return;
-    }
-    float posb;
-    if (a.use_iuc) {
-        float delT = a.uc0 * 1e6 * (a.t + a.dt);
-        float d_delT = a.uc0 * 1e6 * a.dt;
-        posb = nucl_prob(a, delT, d_delT, a.nuc_Nmax0);
-    } else {
-        const float xx = a.x[gi] - a.X0_mac, yy = a.y[gj] - a.Y0_mac, zz = a.z[gk] - a.Z0_mac;
-        float Tc = interp4(a.T3d, xx, yy, zz, a.t - a.t0_mac, a.Nx, a.Ny, a.Nz, a.Nt, a.Dx_mac,
-                           a.Dt_mac);
-        float Tcd = interp4(a.T3d, xx, yy, zz, a.t + a.dt - a.t0_mac, a.Nx, a.Ny, a.Nz, a.Nt,
-                            a.Dx_mac, a.Dt_mac);
-        posb = nucl_prob(a, -Tcd, Tc - Tcd, a.nuc_Nmax);
-    }
-    uint64_t h = mix64(a.seed + 0x9e3779b97f4a7c15ULL);
-    h = mix64(h ^ ((uint64_t)C * 0xd1b54a32d192ed03ULL));
-    h = mix64(h ^ (a.call * 0x8cb92ba72f3d8dd7ULL + 0x632be59bd9b4e019ULL));
-    const float u = (float)((h >> 40) + 1) * (1.0f / 16777216.0f);
-    const int id = 1 + (int)(((h & 0xffffffffULL) * 0x7ffffffeULL) >> 32);
-    if (u < posb) {
-        int q = 0;
-        for (int lk = -a.p; lk <= a.p; ++lk)
-            for (int lj = -a.p; lj <= a.p; ++lj)
-                for (int li = -a.p; li <= a.p; ++li) {
-                    const int o = gC + lk * A + lj * a.fnx + li;
-                    const float v = a.ball[q++];
-                    a.ph[o] = v;
-                    if (v > -0.9999f) a.arg[o] = id;
-                    a.act[(gk + lk) * a.ntiles + act_tile_x(gi + li) + act_tile_y(gj + lj) * a.ntx] =
-                        a.ep;
-                }
-        a.status[C] = 1;
-        atomicAdd(a.events, 1ull);
-    }
-}
-
-// Activity map from a buffer's contents (after it was written other than by
-// a step): one lane per interior point, K slots each.
-__global__ __launch_bounds__(kBlock) void act_rebuild_kernel(const float* __restrict__ ph,
-                                                             const int* __restrict__ arg,
-                                                             unsigned* act, unsigned ep, int fnx,
-                                                             int fny, int fnz, int K, int ntx,
-                                                             int ntiles)
-{
-    const int nx = fnx - 2, ny = fny - 2;
-    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (q >= (long long)nx * ny * (fnz - 2)) return;
-    const int k = 1 + (int)(q / ((long long)nx * ny));
-    const int r = (int)(q - (long long)(k - 1) * nx * ny);
-    const int j = 1 + r / nx;
-    const int i = 1 + r - (j - 1) * nx;
-    const int L = fnx * fny * fnz;
-    const int C = i + j * fnx + k * fnx * fny;
-    bool on = false;
-    for (int s = 0; s < K; ++s) on = on || arg[C + s * L] != -1 || ph[C + s * L] != -1.0f;
-    if (on) act[k * ntiles + act_tile_x(i) + act_tile_y(j) * ntx] = ep;
-}
-
-// Plain streaming copy (16 B per lane per iteration), the reference point for
-// HBM bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters.
-__global__ __launch_bounds__(kBlock) void copy_kernel(const float4* __restrict__ src,
-                                                      float4* __restrict__ dst, long long n)
-{
-    for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < n;
-         q += (long long)gridDim.x * kBlock)
-        dst[q] = src[q];
-}
-
 inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
-
-hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,
-                       hipEvent_t ev_begin, hipEvent_t ev_mid, hipEvent_t ev_end)
-{
-    dim3 grid(a.step_blocks), block(kBlock), fgrid(kFixupBlocks);
-    dim3 pgrid(max(1, cdiv((long long)a.nzchunk * a.items_chunk, kBlock)));
-#define CUPF_LAUNCH(KK, GG)                                                                   \
-    do {                                                                                      \
-        if (ev_begin) (void)hipEventRecord(ev_begin, stream);                                 \
-        hipLaunchKernelGGL(apt_plan, pgrid, block, 0, stream, a);                             \
-        hipLaunchKernelGGL((apt_step_tiled<KK, GG>), grid, block, 0, stream, a);              \
-        if (ev_mid) (void)hipEventRecord(ev_mid, stream);                                     \
-        hipLaunchKernelGGL((apt_fixup<KK, GG>), fgrid, block, 0, stream, a);                  \
-        if (ev_end) (void)hipEventRecord(ev_end, stream);                                     \
-    } while (0)
-    if (!gather27) {
-        if (K == 5) CUPF_LAUNCH(5, false);
-        else if (K == 8) CUPF_LAUNCH(8, false);
-        else return hipErrorInvalidValue;
-    } else {
-        if (K == 5) CUPF_LAUNCH(5, true);
-        else if (K == 8) CUPF_LAUNCH(8, true);
-        else return hipErrorInvalidValue;
-    }
-#undef CUPF_LAUNCH
-    return hipGetLastError();
-}
-
-int step_kernel_slots(int K, int gather27, int device)
-{
-    int ncu = 0, per = 0;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
-        return 0;
-    hipError_t e = hipErrorInvalidValue;
-    if (!gather27 && K == 5)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<5, false>, kBlock, 0);
-    else if (!gather27 && K == 8)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<8, false>, kBlock, 0);
-    else if (gather27 && K == 5)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<5, true>, kBlock, 0);
-    else if (gather27 && K == 8)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<8, true>, kBlock, 0);
-    if (e != hipSuccess || per < 1) return 0;
-    return per * ncu;
-}
-
-hipError_t launch_act_rebuild(const float* ph, const int* arg, unsigned* act, unsigned ep, int fnx,
-                              int fny, int fnz, int K, int ntx, int ntiles, hipStream_t stream)
-{
-    const long long n = (long long)(fnx - 2) * (fny - 2) * (fnz - 2);
-    hipLaunchKernelGGL(act_rebuild_kernel, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, stream, ph, arg,
-                       act, ep, fnx, fny, fnz, K, ntx, ntiles);
-    return hipGetLastError();
-}
-
-hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream)
-{
-    const long long n = (long long)(bytes / 16);
-    hipLaunchKernelGGL(copy_kernel, dim3(256 * 16), dim3(kBlock), 0, stream,
-                       (const float4*)src, (float4*)dst, n);
-    return hipGetLastError();
-}
-
-hipError_t launch_bc_fill(float* ph, int* arg, int fnx, int fny, int fnz, int K, int bcx, int bcy,
-                          int bcz, hipStream_t stream)
-{
-    const int nhalo = 2 * fnx * fny + 2 * fnx * (fnz - 2) + 2 * (fny - 2) * (fnz - 2);
-    dim3 grid(cdiv(nhalo, kBlock), K);
-    hipLaunchKernelGGL(bc_fill, grid, dim3(kBlock), 0, stream, ph, arg, fnx, fny, fnz,
-                       fnx * fny * fnz, bcx, bcy, bcz, nhalo);
-    return hipGetLastError();
-}
-
-hipError_t launch_collect(const float* ph, const int* arg, float* phi, int* alpha, int L, int K,
-                          int clear_alpha, hipStream_t stream)
-{
-    hipLaunchKernelGGL(collect_kernel, dim3(cdiv(L, kBlock)), dim3(kBlock), 0, stream, ph, arg,
-                       phi, alpha, L, K, clear_alpha);
-    return hipGetLastError();
-}
-
-hipError_t launch_ini(float* ph0, int* arg0, float* ph1, int* arg1, const float* phi0,
-                      const int* alpha0, int L, int K, hipStream_t stream)
-{
-    hipLaunchKernelGGL(ini_kernel, dim3(cdiv((long long)L * K, kBlock)), dim3(kBlock), 0, stream,
-                       ph0, arg0, ph1, arg1, phi0, alpha0, L, K);
-    return hipGetLastError();
-}
-
-hipError_t launch_plane_sums(const float* ph, long long* sums, int fnx, int fny, int fnz, int K,
-                             hipStream_t stream)
-{
-    const int A = fnx * fny;
-    hipError_t e = hipMemsetAsync(sums, 0, sizeof(long long) * fnz, stream);
-    if (e != hipSuccess) return e;
-    dim3 grid(min(cdiv(A, kBlock), 64), fnz);
-    hipLaunchKernelGGL(plane_sum_kernel, grid, dim3(kBlock), 0, stream, ph, sums, A, A * fnz, K);
-    return hipGetLastError();
-}
-
-hipError_t launch_move_frame(float* ph_old, int* arg_old, float* ph_new, int* arg_new, float* z,
-                             float* zbuf, const int* alpha, int* alpha_full, int* loss_area,
-                             int move_count, int fnx, int fny, int fnz, int K, int num_theta,
-                             hipStream_t stream)
-{
-    const int L = fnx * fny * fnz;
-    dim3 grid(cdiv((long long)L * K, kBlock));
-    hipLaunchKernelGGL(move_frame_kernel, grid, dim3(kBlock), 0, stream, ph_old, arg_old, ph_new,
-                       arg_new, z, zbuf, alpha, alpha_full, loss_area, move_count, fnx, fny, fnz,
-                       L, K, num_theta);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(copy_frame_kernel, grid, dim3(kBlock), 0, stream, ph_new, arg_new, zbuf,
-                       ph_old, arg_old, z, fnx, fny, fnz, L, K);
-    return hipGetLastError();
-}
-
-hipError_t launch_nucl_status(const float* phi0, int* status, int cnx, int cny, int cnz, int p,
-                              int fnx, int fny, hipStream_t stream)
-{
-    const int n = cnx * cny * cnz;
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(nucl_status_kernel, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, stream, phi0,
-                       status, cnx, cny, cnz, p, fnx, fny);
-    return hipGetLastError();
-}
-
-hipError_t launch_nucl(const NuclArgs& a, hipStream_t stream)
-{
-    const int n = a.cnx * a.cny * a.cnz;
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(nucl_kernel, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, stream, a);
-    return hipGetLastError();
-}
 
 }  // namespace cupf

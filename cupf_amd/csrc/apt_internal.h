@@ -81,6 +81,43 @@ struct StepArgs {
     float X0_mac, Y0_mac, Z0_mac, Dx_mac;
 };
 
+// One simulation of a batch, in device memory.  `a` holds the arguments of
+// its next step; after every step the advance kernel swaps the buffers, moves
+// the epochs and the step index and recomputes t, exactly as the host does
+// between two single-simulation launches.
+struct BatchSim {
+    StepArgs a;
+    float* ph[2];
+    int* arg[2];
+    unsigned* act[2];
+    unsigned ep[2];        // epochs of the two buffers
+    unsigned ep_ctr;       // epoch counter of the simulation
+    int cur;               // source buffer of the next step
+    int step;              // index s of the next step (t = (float)s*dt*tau0)
+    int parity;            // counter set of the next step
+    unsigned* cnt;         // the simulation's two 4-word counter sets
+    float tau0;
+    int plan_base;         // first plan lane of this simulation
+    int edge_base;         // first edge work index of this simulation
+};
+
+// A batched step over n simulations sharing K and the gather variant.  The
+// work list holds tile chunks of every simulation; cnt holds two 4-word
+// counter sets ([0] listed items, [1] next work index) used by alternate
+// steps, selected by cnt[8].
+struct BatchArgs {
+    BatchSim* sims;
+    int n;
+    int plan_total;        // lanes of the plan kernel (all simulations' tile chunks)
+    int edge_total;        // edge work items of all simulations
+    int* wl_sim;
+    int* wl_item;
+    unsigned long long* wl_need;
+    unsigned long long* wl_act;
+    unsigned* cnt;
+    int step_blocks;
+};
+
 // Nucleation launch block (3DgrainMPI/APTPhaseField.cu:459-536).
 struct NuclArgs {
     float* ph;
@@ -106,10 +143,33 @@ struct NuclArgs {
     uint64_t call;
 };
 
-// Launchers (apt_kernels.hip).  All enqueue on `stream` and return the
-// hipError_t of the launch.
+// Launchers (apt_misc.hip, step_k*_g*.hip).  All enqueue on
+// `stream` and return the hipError_t of the launch.
 // Workgroups of the persistent step kernel that fit the device at once.
 int step_kernel_slots(int K, int gather27, int device);
+int batch_kernel_slots(int K, int gather27, int device);
+// Batched step: plan, persistent step, fix-up and advance kernels.
+hipError_t launch_batch_step(const BatchArgs& b, int K, int gather27, hipStream_t stream);
+// Pieces shared by the per-K launchers.
+hipError_t launch_plan(const StepArgs& a, hipStream_t stream);
+hipError_t launch_batch_plan(const BatchArgs& b, hipStream_t stream);
+hipError_t launch_batch_advance(const BatchArgs& b, hipStream_t stream);
+hipError_t launch_step_k5_g7(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
+                             hipEvent_t e2);
+hipError_t launch_batch_k5_g7(const BatchArgs& b, hipStream_t stream);
+int step_slots_k5_g7(int device, bool batch);
+hipError_t launch_step_k5_g27(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
+                             hipEvent_t e2);
+hipError_t launch_batch_k5_g27(const BatchArgs& b, hipStream_t stream);
+int step_slots_k5_g27(int device, bool batch);
+hipError_t launch_step_k8_g7(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
+                             hipEvent_t e2);
+hipError_t launch_batch_k8_g7(const BatchArgs& b, hipStream_t stream);
+int step_slots_k8_g7(int device, bool batch);
+hipError_t launch_step_k8_g27(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
+                             hipEvent_t e2);
+hipError_t launch_batch_k8_g27(const BatchArgs& b, hipStream_t stream);
+int step_slots_k8_g27(int device, bool batch);
 // Main step kernel then the overflow fix-up kernel.  Optional events are
 // recorded before the main kernel, between the two and after the fix-up.
 hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,

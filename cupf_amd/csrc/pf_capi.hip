@@ -6,6 +6,7 @@
 // work goes to the context's own HIP stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -256,10 +257,11 @@ hipError_t act_rebuild(pf_ctx* c, int b)
                               c->ntx, c->ntiles, c->stream);
 }
 
-// Before the epoch counter could wrap: clear both maps and rebuild them.
-hipError_t act_renew_if_needed(pf_ctx* c)
+// Before the epoch counter could wrap within the next `ahead` steps: clear
+// both maps and rebuild them.
+hipError_t act_renew_if_needed(pf_ctx* c, unsigned ahead = 1u)
 {
-    if (c->ep_ctr < 0x7fffffffu) return hipSuccess;
+    if (c->ep_ctr < 0x7fffffffu - ahead) return hipSuccess;
     const size_t n = (size_t)c->ntiles * c->p.fnz;
     hipError_t e;
     for (int b = 0; b < 2; ++b)
@@ -283,6 +285,153 @@ void free_ctx(pf_ctx* c)
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+}  // namespace
+
+// A batch: several contexts of one device stepped together (see pf_batch_step).
+struct pf_batch {
+    std::vector<pf_ctx*> ctxs;
+    int device = 0, K = 0, g27 = 0;
+    hipStream_t stream = nullptr;
+    int kchunk = 4;
+    int step_blocks = 0;
+    BatchSim* sims = nullptr;       // device array, n entries
+    int* wl_sim = nullptr;
+    int* wl_item = nullptr;
+    unsigned long long* wl_need = nullptr;
+    unsigned long long* wl_act = nullptr;
+    unsigned* cnt = nullptr;        // 2 x 4 list counters + parity word
+    size_t wl_cap = 0;
+    // hipGraph of kGraphSteps batched steps, captured for one kchunk
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_kchunk = 0;
+    static constexpr int kGraphSteps = 8;
+    static constexpr int kMinChunk = 2;
+};
+
+namespace {
+
+void batch_free(pf_batch* b)
+{
+    (void)hipSetDevice(b->device);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    void* bufs[] = {b->sims, b->wl_sim, b->wl_item, b->wl_need, b->wl_act, b->cnt};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// Step arguments of every member for steps first_step[i].., with the batch's
+// chunk depth, as device BatchSim records (and the work-index bases).
+int batch_upload(pf_batch* b, const std::vector<int>& first_step, int use_iuc, BatchArgs& ba)
+{
+    const int n = (int)b->ctxs.size();
+    std::vector<BatchSim> h(n);
+    int plan_total = 0, edge_total = 0;
+    for (int i = 0; i < n; ++i) {
+        pf_ctx* c = b->ctxs[i];
+        const pf_params& p = c->p;
+        const float t = first_step[i] * p.dt * p.tau0;
+        BatchSim& m = h[i];
+        m.a = make_step_args(c, c->cur, t, use_iuc);  // takes this step's epoch from the context
+        m.a.kchunk = b->kchunk;
+        m.a.nzchunk = (p.fnz - 2 + b->kchunk - 1) / b->kchunk;
+        m.a.edge_items = (int)(((long long)b->kchunk * c->edge_cols + 255) / 256);
+        m.a.step_blocks = b->step_blocks;
+        for (int q = 0; q < 2; ++q) {
+            m.ph[q] = c->ph[q];
+            m.arg[q] = c->arg[q];
+            m.act[q] = c->act[q];
+            m.ep[q] = c->ep[q];
+        }
+        m.ep_ctr = c->ep_ctr;
+        m.cur = c->cur;
+        m.step = first_step[i];
+        m.parity = (int)(c->nlaunch & 1ull);
+        m.cnt = c->fix_count;
+        m.tau0 = p.tau0;
+        m.plan_base = plan_total;
+        m.edge_base = edge_total;
+        plan_total += m.a.nzchunk * m.a.items_chunk;
+        edge_total += m.a.nzchunk * m.a.edge_items;
+    }
+    HIPCHK(hipMemcpyAsync(b->sims, h.data(), sizeof(BatchSim) * n, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemsetAsync(b->cnt, 0, 9 * sizeof(unsigned), b->stream));
+    ba.sims = b->sims;
+    ba.n = n;
+    ba.plan_total = plan_total;
+    ba.edge_total = edge_total;
+    ba.wl_sim = b->wl_sim;
+    ba.wl_item = b->wl_item;
+    ba.wl_need = b->wl_need;
+    ba.wl_act = b->wl_act;
+    ba.cnt = b->cnt;
+    ba.step_blocks = b->step_blocks;
+    return PF_OK;
+}
+
+// The host side of nsteps batched steps (what step_impl does per launch),
+// after make_step_args already took the first step's epoch.
+void batch_mirror(pf_batch* b, int nsteps)
+{
+    for (pf_ctx* c : b->ctxs) {
+        for (int s = 0; s < nsteps; ++s) {
+            if (s > 0) ++c->ep_ctr;
+            c->ep[1 - c->cur] = c->ep_ctr;
+            c->cur = 1 - c->cur;
+            ++c->nlaunch;
+        }
+    }
+}
+
+// nsteps batched steps of every member from first_step (advanced on return).
+int batch_run(pf_batch* b, int nsteps, std::vector<int>& first_step, int use_iuc, bool use_graph)
+{
+    if (nsteps <= 0) return PF_OK;
+    for (pf_ctx* c : b->ctxs) {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(act_renew_if_needed(c, (unsigned)nsteps + 2u));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    BatchArgs ba;
+    int rc = batch_upload(b, first_step, use_iuc, ba);
+    if (rc) return rc;
+    int left = nsteps;
+    if (use_graph && nsteps >= pf_batch::kGraphSteps) {
+        if (!b->gexec || b->graph_kchunk != b->kchunk) {
+            if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->gexec = nullptr;
+            b->graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for (int q = 0; q < pf_batch::kGraphSteps && e == hipSuccess; ++q)
+                e = launch_batch_step(ba, b->K, b->g27, b->stream);
+            hipGraph_t g = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(PF_E_HIP, std::string("graph capture: ") +
+                                          hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            b->graph = g;
+            HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+            b->graph_kchunk = b->kchunk;
+        }
+        while (left >= pf_batch::kGraphSteps) {
+            HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+            left -= pf_batch::kGraphSteps;
+        }
+    }
+    for (; left > 0; --left) HIPCHK(launch_batch_step(ba, b->K, b->g27, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    batch_mirror(b, nsteps);
+    for (int& s : first_step) s += nsteps;
+    return PF_OK;
 }
 
 }  // namespace
@@ -864,6 +1013,124 @@ int pf_activity(pf_ctx* c, int which, unsigned char* map, long long* active, lon
     if (active) *active = n;
     if (total) *total = (long long)c->ntiles * (p.fnz - 2);
     return PF_OK;
+}
+
+int pf_batch_create(pf_ctx* const* ctxs, int n, pf_batch** out)
+{
+    if (!ctxs || !out || n < 1) return fail(PF_E_INVALID, "bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return fail(PF_E_INVALID, "null context in batch");
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) return fail(PF_E_INVALID, "context listed twice in batch");
+        if (ctxs[i]->device != ctxs[0]->device || ctxs[i]->p.K != ctxs[0]->p.K ||
+            ctxs[i]->p.gather27 != ctxs[0]->p.gather27)
+            return fail(PF_E_INVALID, "batch members must share device, K and gather variant");
+    }
+    HIPCHK(hipSetDevice(ctxs[0]->device));
+    pf_batch* b = new pf_batch();
+    b->ctxs.assign(ctxs, ctxs + n);
+    b->device = ctxs[0]->device;
+    b->K = ctxs[0]->p.K;
+    b->g27 = ctxs[0]->p.gather27;
+    if (const char* kc = getenv("CUPF_KCHUNK")) {
+        const int v = atoi(kc);
+        if (v >= pf_batch::kMinChunk && v <= 61) b->kchunk = v;
+    }
+    b->step_blocks = batch_kernel_slots(b->K, b->g27, b->device);
+    auto bail = [&](int code) {
+        batch_free(b);
+        return code;
+    };
+    if (b->step_blocks < 1) return bail(fail(PF_E_HIP, "occupancy query for the batch kernel failed"));
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(PF_E_HIP, "hipStreamCreate failed"));
+    for (pf_ctx* c : b->ctxs)
+        b->wl_cap += (size_t)((c->p.fnz - 2 + pf_batch::kMinChunk - 1) / pf_batch::kMinChunk) *
+                     c->items_chunk;
+    int rc;
+    if ((rc = dalloc(&b->sims, (size_t)n)) || (rc = dalloc(&b->wl_sim, b->wl_cap + 1)) ||
+        (rc = dalloc(&b->wl_item, b->wl_cap + 1)) || (rc = dalloc(&b->wl_need, b->wl_cap + 1)) ||
+        (rc = dalloc(&b->wl_act, b->wl_cap + 1)) || (rc = dalloc(&b->cnt, 9)))
+        return bail(rc);
+    *out = b;
+    return PF_OK;
+}
+
+int pf_batch_destroy(pf_batch* b)
+{
+    if (!b) return PF_OK;
+    (void)hipStreamSynchronize(b->stream);
+    batch_free(b);
+    return PF_OK;
+}
+
+int pf_batch_set_kchunk(pf_batch* b, int kchunk)
+{
+    if (!b) return fail(PF_E_INVALID, "null batch");
+    if (kchunk < pf_batch::kMinChunk || kchunk > 61) return fail(PF_E_INVALID, "kchunk must be 2..61");
+    b->kchunk = kchunk;
+    return PF_OK;
+}
+
+int pf_batch_get_kchunk(pf_batch* b, int* kchunk)
+{
+    if (!b || !kchunk) return fail(PF_E_INVALID, "null argument");
+    *kchunk = b->kchunk;
+    return PF_OK;
+}
+
+int pf_batch_step(pf_batch* b, int nsteps, const int* first_step, int flags)
+{
+    if (!b || !first_step) return fail(PF_E_INVALID, "null argument");
+    if (nsteps < 0) return fail(PF_E_INVALID, "nsteps must be >= 0");
+    if (flags & PF_STEP_NUCLEATE) return fail(PF_E_INVALID, "batched steps exclude nucleation");
+    for (pf_ctx* c : b->ctxs) {
+        if (!c->have_coords || !c->have_orient || !c->have_state)
+            return fail(PF_E_STATE, "batch member needs coordinates, orientations and a state");
+        if (c->p.thermal_type == PF_THERMAL_PARABOLOID && !c->psi_mac)
+            return fail(PF_E_STATE, "paraboloid thermal field needs psi_mac");
+    }
+    HIPCHK(hipSetDevice(b->device));
+    const int use_iuc = (flags & PF_STEP_INITIAL_UC) ? 1 : 0;
+    const bool use_graph = (flags & PF_BATCH_GRAPH) != 0;
+    std::vector<int> fs(first_step, first_step + b->ctxs.size());
+    int left = nsteps;
+    if ((flags & PF_BATCH_AUTOTUNE) && nsteps >= 64) {
+        // the chunk depth changes only how the work is cut, never the results:
+        // time a few real steps with each candidate and keep the fastest
+        const int cand[] = {2, 4, 8, 16};
+        const int m = std::min(16, nsteps / 8);
+        float best = 0.0f;
+        int best_k = b->kchunk;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        int rc = PF_OK;
+        for (int kc : cand) {
+            b->kchunk = kc;
+            if ((rc = batch_run(b, 1, fs, use_iuc, false))) break;  // warm the chunk depth
+            left -= 1;
+            if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(PF_E_HIP, "event"); break; }
+            if ((rc = batch_run(b, m, fs, use_iuc, false))) break;
+            left -= m;
+            float ms = 0.0f;
+            if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = fail(PF_E_HIP, "autotune timing");
+                break;
+            }
+            if (best == 0.0f || ms < best) {
+                best = ms;
+                best_k = kc;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        b->kchunk = best_k;
+    }
+    return batch_run(b, left, fs, use_iuc, use_graph);
 }
 
 int pf_sync(pf_ctx* c)

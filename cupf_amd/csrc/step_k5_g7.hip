@@ -1,0 +1,19 @@
+// step_k5_g7.hip -- step and fix-up kernels for K=5 active slots, 7-point id gather.
+#include "step_kernels.h"
+
+namespace cupf {
+
+hipError_t launch_step_k5_g7(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
+                             hipEvent_t e2)
+{
+    return launch_step_t<5, false>(a, stream, e0, e1, e2);
+}
+
+hipError_t launch_batch_k5_g7(const BatchArgs& b, hipStream_t stream)
+{
+    return launch_batch_t<5, false>(b, stream);
+}
+
+int step_slots_k5_g7(int device, bool batch) { return slots_t<5, false>(device, batch); }
+
+}  // namespace cupf

@@ -1,0 +1,19 @@
+// step_k8_g27.hip -- step and fix-up kernels for K=8 active slots, 27-point id gather.
+#include "step_kernels.h"
+
+namespace cupf {
+
+hipError_t launch_step_k8_g27(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
+                             hipEvent_t e2)
+{
+    return launch_step_t<8, true>(a, stream, e0, e1, e2);
+}
+
+hipError_t launch_batch_k8_g27(const BatchArgs& b, hipStream_t stream)
+{
+    return launch_batch_t<8, true>(b, stream);
+}
+
+int step_slots_k8_g27(int device, bool batch) { return slots_t<8, true>(device, batch); }
+
+}  // namespace cupf

@@ -1,0 +1,200 @@
+// step_kernels.h -- the persistent step and fix-up kernels (single simulation
+// and batched), instantiated once per (K, gather variant) by the step_k*_g*.hip
+// files so the large template instantiations compile in parallel.
+#pragma once
+
+#include "apt_device.h"
+
+namespace cupf {
+
+namespace {
+
+// APTrhs_psi (3DgrainMPI/APTPhaseField.cu:178-354) over every interior point:
+// a persistent grid of step_blocks workgroups takes work items one at a time,
+// the edge items first (direct path, slowest per point), then the listed tile
+// chunks.  Every workgroup leaves once the items are exhausted; nothing waits
+// on another workgroup.
+template <int K, bool G27>
+__global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
+{
+    __shared__ unsigned s_q;
+    const float a_s = 1.0f + 3.0f * a.kin_delta;
+    const float epsv = -4.0f * a.kin_delta / a_s;
+    const unsigned ne = (unsigned)(a.nzchunk * a.edge_items);
+    const unsigned n = ne + __atomic_load_n(a.wl_count, __ATOMIC_RELAXED);
+    for (;;) {
+        // the next item is taken only when this workgroup is free, which keeps
+        // the tail of the list balanced
+        if (threadIdx.x == 0) s_q = atomicAdd(a.wl_count + 1, 1u);
+        __syncthreads();
+        const unsigned q = s_q;
+        __syncthreads();
+        if (q >= n) break;
+        if (q < ne) {
+            const int zc = (int)q / a.edge_items;
+            const int e = (int)q - zc * a.edge_items;
+            const int k0 = 1 + zc * a.kchunk;
+            const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+            edge_point<K, G27>(a, e * kBlock + threadIdx.x, k0, k1, a_s, epsv);
+        } else {
+            const unsigned w = q - ne;
+            const int g = a.wl_item[w];
+            const int zc = g / a.items_chunk;
+            const int t = g - zc * a.items_chunk;
+            const int ty = t / a.tiles_x;
+            const int tx = t - ty * a.tiles_x;
+            const int k0 = 1 + zc * a.kchunk;
+            const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+            tile_march<K, G27>(a, tx, ty, k0, k1, a.wl_need[w], a.wl_act[w], a_s, epsv);
+        }
+    }
+}
+
+// Fix-up pass for the points of the last apt_step launch whose neighbourhood
+// overflowed K slots; adds their number to the overflow diagnostic.  The list
+// lives in this step's counter set, which the next-but-one plan kernel
+// clears, so nothing here has to wait for the other workgroups.
+template <int K, bool G27>
+__global__ __launch_bounds__(kBlock) void apt_fixup(StepArgs a)
+{
+    const unsigned n = __atomic_load_n(a.fix_count, __ATOMIC_RELAXED);
+    if (n == 0u) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.overflow, (unsigned long long)n);
+    const float a_s = 1.0f + 3.0f * a.kin_delta;
+    const float epsv = -4.0f * a.kin_delta / a_s;
+    const int A = a.fnx * a.fny;
+    for (unsigned q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+        const int C = (int)a.fix_list[q];
+        const int k = C / A;
+        const int r = C - k * A;
+        const int j = r / a.fnx;
+        const int i = r - j * a.fnx;
+        apt_point_overflow<K, G27>(a, i, j, k, a_s, epsv);
+    }
+}
+
+// Index of the simulation whose range [base_i, base_{i+1}) holds work index q
+// (bases ascending; sims with an empty range are skipped over).
+__device__ __forceinline__ int batch_find(const BatchSim* sims, int n, int q, bool edge)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int base = edge ? sims[mid].edge_base : sims[mid].plan_base;
+        if (base <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Batched APTrhs_psi: the persistent grid takes the edge items of every
+// simulation, then the tile chunks the batched plan kernel listed, each with
+// its own simulation's step arguments.
+template <int K, bool G27>
+__global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
+{
+    __shared__ unsigned s_q;
+    const unsigned* cnt = b.cnt + 4 * __atomic_load_n(b.cnt + 8, __ATOMIC_RELAXED);
+    const unsigned ne = (unsigned)b.edge_total;
+    const unsigned n = ne + __atomic_load_n(cnt, __ATOMIC_RELAXED);
+    for (;;) {
+        if (threadIdx.x == 0) s_q = atomicAdd((unsigned*)cnt + 1, 1u);
+        __syncthreads();
+        const unsigned q = s_q;
+        __syncthreads();
+        if (q >= n) break;
+        if (q < ne) {
+            const int sim = batch_find(b.sims, b.n, (int)q, true);
+            const StepArgs a = b.sims[sim].a;
+            const int e0 = (int)q - b.sims[sim].edge_base;
+            const int zc = e0 / a.edge_items;
+            const int e = e0 - zc * a.edge_items;
+            const int k0 = 1 + zc * a.kchunk;
+            const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+            const float a_s = 1.0f + 3.0f * a.kin_delta;
+            const float epsv = -4.0f * a.kin_delta / a_s;
+            edge_point<K, G27>(a, e * kBlock + threadIdx.x, k0, k1, a_s, epsv);
+        } else {
+            const unsigned w = q - ne;
+            const StepArgs a = b.sims[b.wl_sim[w]].a;
+            const int g = b.wl_item[w];
+            const int zc = g / a.items_chunk;
+            const int t = g - zc * a.items_chunk;
+            const int ty = t / a.tiles_x;
+            const int tx = t - ty * a.tiles_x;
+            const int k0 = 1 + zc * a.kchunk;
+            const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+            const float a_s = 1.0f + 3.0f * a.kin_delta;
+            const float epsv = -4.0f * a.kin_delta / a_s;
+            tile_march<K, G27>(a, tx, ty, k0, k1, b.wl_need[w], b.wl_act[w], a_s, epsv);
+        }
+    }
+}
+
+// Batched fix-up: every simulation's overflow list of this step in turn.
+template <int K, bool G27>
+__global__ __launch_bounds__(kBlock) void batch_fixup(BatchArgs b)
+{
+    for (int sim = 0; sim < b.n; ++sim) {
+        const StepArgs a = b.sims[sim].a;
+        const unsigned n = __atomic_load_n(a.fix_count, __ATOMIC_RELAXED);
+        if (n == 0u) continue;
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.overflow, (unsigned long long)n);
+        const float a_s = 1.0f + 3.0f * a.kin_delta;
+        const float epsv = -4.0f * a.kin_delta / a_s;
+        const int A = a.fnx * a.fny;
+        for (unsigned q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+            const int C = (int)a.fix_list[q];
+            const int k = C / A;
+            const int r = C - k * A;
+            const int j = r / a.fnx;
+            const int i = r - j * a.fnx;
+            apt_point_overflow<K, G27>(a, i, j, k, a_s, epsv);
+        }
+    }
+}
+
+template <int K, bool G27>
+hipError_t launch_step_t(const StepArgs& a, hipStream_t stream, hipEvent_t ev_begin,
+                         hipEvent_t ev_mid, hipEvent_t ev_end)
+{
+    dim3 grid(a.step_blocks), block(kBlock), fgrid(kFixupBlocks);
+    if (ev_begin) (void)hipEventRecord(ev_begin, stream);
+    hipError_t e = launch_plan(a, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((apt_step_tiled<K, G27>), grid, block, 0, stream, a);
+    if (ev_mid) (void)hipEventRecord(ev_mid, stream);
+    hipLaunchKernelGGL((apt_fixup<K, G27>), fgrid, block, 0, stream, a);
+    if (ev_end) (void)hipEventRecord(ev_end, stream);
+    return hipGetLastError();
+}
+
+template <int K, bool G27>
+hipError_t launch_batch_t(const BatchArgs& b, hipStream_t stream)
+{
+    dim3 grid(b.step_blocks), block(kBlock), fgrid(kFixupBlocks);
+    hipError_t e = launch_batch_plan(b, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((batch_step<K, G27>), grid, block, 0, stream, b);
+    hipLaunchKernelGGL((batch_fixup<K, G27>), fgrid, block, 0, stream, b);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_batch_advance(b, stream);
+}
+
+template <int K, bool G27>
+int slots_t(int device, bool batch)
+{
+    int ncu = 0, per = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0;
+    const hipError_t e =
+        batch ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_step<K, G27>, kBlock, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<K, G27>, kBlock, 0);
+    if (e != hipSuccess || per < 1) return 0;
+    return per * ncu;
+}
+
+}  // namespace
+
+}  // namespace cupf

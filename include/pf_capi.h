@@ -187,6 +187,33 @@ int pf_sync(pf_ctx* ctx);
  * variable CUPF_NO_ACTIVITY=1 at pf_create turns the skipping off. */
 int pf_activity(pf_ctx* ctx, int which, unsigned char* map, long long* active, long long* total);
 
+/* ---- Batched ensembles --------------------------------------------------
+ * No counterpart in the reference, which runs one simulation per process
+ * (3Dgrain/createcmd:1 launches them one after another).  A batch steps
+ * several contexts of one device (same K and gather variant; grids, thermal
+ * parameters and seeds may differ) with one plan, one persistent step, one
+ * fix-up and one advance kernel per time step over all of them, so small
+ * simulations share the GPU instead of each being launch-bound.  Every member
+ * gets exactly the bits pf_step would give it: batching changes the schedule,
+ * not the arithmetic.  Members stay ordinary contexts: set them up, upload,
+ * download, collect and step them singly as before; a batch only borrows
+ * them during pf_batch_step. */
+typedef struct pf_batch pf_batch;
+
+/* pf_batch_step flags (beside PF_STEP_INITIAL_UC) */
+#define PF_BATCH_GRAPH 4      /* replay groups of 8 steps from one captured hipGraph */
+#define PF_BATCH_AUTOTUNE 8   /* time a few real steps with z-chunk depths 2, 4, 8, 16
+                                 and keep the fastest (needs nsteps >= 64) */
+
+int pf_batch_create(pf_ctx* const* ctxs, int n, pf_batch** out);
+int pf_batch_destroy(pf_batch* b);
+/* nsteps steps of every member; member i runs steps first_step[i] ..
+ * first_step[i]+nsteps-1 (as pf_step's first_step).  No nucleation. */
+int pf_batch_step(pf_batch* b, int nsteps, const int* first_step, int flags);
+/* z planes per work item (2..61); default 4, or CUPF_KCHUNK at creation. */
+int pf_batch_set_kchunk(pf_batch* b, int kchunk);
+int pf_batch_get_kchunk(pf_batch* b, int* kchunk);
+
 /* Diagnostic: bandwidth of a plain device-to-device copy kernel over `bytes`
  * (read + write counted), iters launches; the reference point for the HBM
  * roofline and for scaling HBM byte counters.  No counterpart in the reference. */
