@@ -101,6 +101,9 @@ struct BatchSim {
     int edge_base;         // first edge work index of this simulation
 };
 
+// Most simulations one batch may hold.
+constexpr int kMaxBatch = 1024;
+
 // A batched step over n simulations sharing K and the gather variant.  The
 // work list holds tile chunks of every simulation; cnt holds two 4-word
 // counter sets ([0] listed items, [1] next work index) used by alternate

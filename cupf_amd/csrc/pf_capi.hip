@@ -1018,6 +1018,7 @@ int pf_activity(pf_ctx* c, int which, unsigned char* map, long long* active, lon
 int pf_batch_create(pf_ctx* const* ctxs, int n, pf_batch** out)
 {
     if (!ctxs || !out || n < 1) return fail(PF_E_INVALID, "bad arguments");
+    if (n > kMaxBatch) return fail(PF_E_INVALID, "a batch holds at most 1024 simulations");
     *out = nullptr;
     for (int i = 0; i < n; ++i) {
         if (!ctxs[i]) return fail(PF_E_INVALID, "null context in batch");

@@ -3,6 +3,8 @@
 // files so the large template instantiations compile in parallel.
 #pragma once
 
+#include <algorithm>
+
 #include "apt_device.h"
 
 namespace cupf {
@@ -94,6 +96,12 @@ template <int K, bool G27>
 __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
 {
     __shared__ unsigned s_q;
+    // the current item's step arguments, staged once per item: the code reads
+    // its fields from LDS (uniform addresses) instead of every lane copying the
+    // whole block from global memory into registers
+    __shared__ StepArgs s_a;
+    constexpr int kWords = (int)(sizeof(StepArgs) / 4);
+    static_assert(sizeof(StepArgs) % 4 == 0 && kWords <= kBlock, "StepArgs staging");
     const unsigned* cnt = b.cnt + 4 * __atomic_load_n(b.cnt + 8, __ATOMIC_RELAXED);
     const unsigned ne = (unsigned)b.edge_total;
     const unsigned n = ne + __atomic_load_n(cnt, __ATOMIC_RELAXED);
@@ -101,22 +109,25 @@ __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
         if (threadIdx.x == 0) s_q = atomicAdd((unsigned*)cnt + 1, 1u);
         __syncthreads();
         const unsigned q = s_q;
-        __syncthreads();
         if (q >= n) break;
-        if (q < ne) {
-            const int sim = batch_find(b.sims, b.n, (int)q, true);
-            const StepArgs a = b.sims[sim].a;
+        const bool edge = q < ne;
+        const unsigned w = q - ne;
+        const int sim = edge ? batch_find(b.sims, b.n, (int)q, true) : b.wl_sim[w];
+        if ((int)threadIdx.x < kWords)
+            reinterpret_cast<unsigned*>(&s_a)[threadIdx.x] =
+                reinterpret_cast<const unsigned*>(&b.sims[sim].a)[threadIdx.x];
+        __syncthreads();
+        const StepArgs& a = s_a;
+        const float a_s = 1.0f + 3.0f * a.kin_delta;
+        const float epsv = -4.0f * a.kin_delta / a_s;
+        if (edge) {
             const int e0 = (int)q - b.sims[sim].edge_base;
             const int zc = e0 / a.edge_items;
             const int e = e0 - zc * a.edge_items;
             const int k0 = 1 + zc * a.kchunk;
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-            const float a_s = 1.0f + 3.0f * a.kin_delta;
-            const float epsv = -4.0f * a.kin_delta / a_s;
             edge_point<K, G27>(a, e * kBlock + threadIdx.x, k0, k1, a_s, epsv);
         } else {
-            const unsigned w = q - ne;
-            const StepArgs a = b.sims[b.wl_sim[w]].a;
             const int g = b.wl_item[w];
             const int zc = g / a.items_chunk;
             const int t = g - zc * a.items_chunk;
@@ -124,33 +135,49 @@ __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
             const int tx = t - ty * a.tiles_x;
             const int k0 = 1 + zc * a.kchunk;
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-            const float a_s = 1.0f + 3.0f * a.kin_delta;
-            const float epsv = -4.0f * a.kin_delta / a_s;
             tile_march<K, G27>(a, tx, ty, k0, k1, b.wl_need[w], b.wl_act[w], a_s, epsv);
         }
+        __syncthreads();  // s_q and s_a are rewritten for the next item
     }
 }
 
-// Batched fix-up: every simulation's overflow list of this step in turn.
+// Batched fix-up: the overflow lists of all simulations form one index space
+// (prefix sums of their lengths, in LDS) that the grid strides over, so long
+// and short lists alike are spread over every workgroup.
 template <int K, bool G27>
 __global__ __launch_bounds__(kBlock) void batch_fixup(BatchArgs b)
 {
-    for (int sim = 0; sim < b.n; ++sim) {
-        const StepArgs a = b.sims[sim].a;
-        const unsigned n = __atomic_load_n(a.fix_count, __ATOMIC_RELAXED);
-        if (n == 0u) continue;
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.overflow, (unsigned long long)n);
+    __shared__ unsigned s_pre[kMaxBatch + 1];
+    for (int t = threadIdx.x; t < b.n; t += kBlock)
+        s_pre[t + 1] = __atomic_load_n(b.sims[t].a.fix_count, __ATOMIC_RELAXED);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_pre[0] = 0u;
+        for (int t = 0; t < b.n; ++t) {
+            const unsigned c = s_pre[t + 1];
+            if (blockIdx.x == 0 && c) atomicAdd(b.sims[t].a.overflow, (unsigned long long)c);
+            s_pre[t + 1] = s_pre[t] + c;
+        }
+    }
+    __syncthreads();
+    const unsigned total = s_pre[b.n];
+    for (unsigned g = blockIdx.x * kBlock + threadIdx.x; g < total; g += gridDim.x * kBlock) {
+        int lo = 0, hi = b.n - 1;  // last simulation whose range starts at or before g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_pre[mid] <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        const StepArgs& a = b.sims[lo].a;
+        const int C = (int)a.fix_list[g - s_pre[lo]];
+        const int A = a.fnx * a.fny;
+        const int k = C / A;
+        const int r = C - k * A;
+        const int j = r / a.fnx;
+        const int i = r - j * a.fnx;
         const float a_s = 1.0f + 3.0f * a.kin_delta;
         const float epsv = -4.0f * a.kin_delta / a_s;
-        const int A = a.fnx * a.fny;
-        for (unsigned q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
-            const int C = (int)a.fix_list[q];
-            const int k = C / A;
-            const int r = C - k * A;
-            const int j = r / a.fnx;
-            const int i = r - j * a.fnx;
-            apt_point_overflow<K, G27>(a, i, j, k, a_s, epsv);
-        }
+        apt_point_overflow<K, G27>(a, i, j, k, a_s, epsv);
     }
 }
 
@@ -172,11 +199,11 @@ hipError_t launch_step_t(const StepArgs& a, hipStream_t stream, hipEvent_t ev_be
 template <int K, bool G27>
 hipError_t launch_batch_t(const BatchArgs& b, hipStream_t stream)
 {
-    dim3 grid(b.step_blocks), block(kBlock), fgrid(kFixupBlocks);
+    dim3 grid(b.step_blocks), block(kBlock);
     hipError_t e = launch_batch_plan(b, stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((batch_step<K, G27>), grid, block, 0, stream, b);
-    hipLaunchKernelGGL((batch_fixup<K, G27>), fgrid, block, 0, stream, b);
+    hipLaunchKernelGGL((batch_fixup<K, G27>), dim3(kFixupBlocks), block, 0, stream, b);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_batch_advance(b, stream);

@@ -205,6 +205,7 @@ typedef struct pf_batch pf_batch;
 #define PF_BATCH_AUTOTUNE 8   /* time a few real steps with z-chunk depths 2, 4, 8, 16
                                  and keep the fastest (needs nsteps >= 64) */
 
+/* n <= 1024 distinct contexts. */
 int pf_batch_create(pf_ctx* const* ctxs, int n, pf_batch** out);
 int pf_batch_destroy(pf_batch* b);
 /* nsteps steps of every member; member i runs steps first_step[i] ..
