@@ -18,6 +18,16 @@ Prints one JSON line:
              updated points (the dense count) is given as dense_equiv_gbs.
   cpu_baseline  the CPU oracle (float32 restatement, OpenMP) on the same
              post-warmup state for a few steps
+  ensemble   64 seeded simulations, grids drawn from 32^3, 64^3, 96x96x128 and
+             128x128x256, G log-uniform in [1e5, 1e7] K/m, R log-uniform in
+             [0.01, 1] m/s, K=8, 2000 steps each: batched (one launch sequence
+             per step for all of them, hipGraph replay, autotuned chunk depth)
+             against back-to-back single runs on the same GPU; both stepping
+             times, sims/h, and whether every simulation ended with the same bits
+  serving    a seeded Poisson stream (4 requests/s) of the ensemble's
+             distributions with 500..5000 steps each, continuously batched
+             under an HBM budget: sims/h and p50/p99 latency from arrival to
+             retirement
 """
 from __future__ import annotations
 
@@ -91,6 +101,57 @@ def processed_points(ctx, p) -> int:
     return int((proc[1:fnz - 1] * pts[None]).sum())
 
 
+def ensemble_block(count: int, K: int) -> dict:
+    from cupf_amd import ensemble
+    specs = ensemble.ensemble_specs(count, seed=0, nsteps=2000)
+    t0 = time.time()
+    prepared = [ensemble.make_case(s, K) for s in specs]
+    log(f"[bench] ensemble: {count} sims prepared in {time.time() - t0:.1f}s, "
+        f"{sum(s.footprint(K) for s in specs) / 1e9:.1f} GB of device state")
+    tb, kc, hb = ensemble.run_batched(specs, prepared, K=K, graph=True, autotune=True)
+    log(f"[bench] ensemble batched: {tb:.2f} s (chunk depth {kc})")
+    ta, each, ha = ensemble.run_back_to_back(specs, prepared, K=K)
+    log(f"[bench] ensemble back-to-back: {ta:.2f} s; same bits: {ha == hb}")
+    grids = {}
+    for s in specs:
+        grids["x".join(map(str, s.n))] = grids.get("x".join(map(str, s.n)), 0) + 1
+    return {
+        "workload": f"{count} sims x 2000 steps, K={K}",
+        "grids": grids,
+        "sims_per_h": round(count * 3600.0 / tb, 1),
+        "unit": "sims/h",
+        "batched_s": round(tb, 3),
+        "back_to_back_s": round(ta, 3),
+        "back_to_back_sims_per_h": round(count * 3600.0 / ta, 1),
+        "speedup_vs_back_to_back": round(ta / tb, 3),
+        "chunk_depth": kc,
+        "identical_results": ha == hb,
+        "note": "stepping time only (inputs prepared and contexts initialised beforehand, in both)",
+    }
+
+
+def serving_block(count: int, K: int) -> dict:
+    from cupf_amd import ensemble
+    specs = ensemble.request_stream(count, rate=4.0, seed=1)
+    prepared = [ensemble.make_case(s, K) for s in specs]
+    srv = ensemble.Server(hbm_budget=64 << 30, quantum=200, K=K)
+    done, wall = srv.run(specs, prepared)
+    lat = [d.latency for d in done]
+    log(f"[bench] serving: {len(done)} requests in {wall:.2f} s, p50 {ensemble.percentile(lat, 50):.3f} s, "
+        f"p99 {ensemble.percentile(lat, 99):.3f} s")
+    return {
+        "workload": f"{count} requests, Poisson 4/s, 500..5000 steps, ensemble grids, K={K}",
+        "sims_per_h": round(len(done) * 3600.0 / wall, 1),
+        "p50_latency_s": round(ensemble.percentile(lat, 50), 3),
+        "p99_latency_s": round(ensemble.percentile(lat, 99), 3),
+        "wall_s": round(wall, 3),
+        "sim_steps": int(sum(s.nsteps for s in specs)),
+        "hbm_budget_gb": 64,
+        "note": "latency from arrival to retirement; arrivals span "
+                f"{specs[-1].arrival:.1f} s, so sims/h is bounded by the arrival rate",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +164,10 @@ def main() -> int:
                     help="only time a plain copy kernel over this many bytes (counter calibration)")
     ap.add_argument("--z0-cells", type=float, default=None,
                     help="initial interface height in cells (default: z0 = 2 um as in line.py)")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--ensemble-sims", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -188,8 +253,15 @@ def main() -> int:
         }
         log(f"[bench] cpu oracle: {cpu['value']} Mcell-steps/s on {threads} threads")
 
+    ens = None
+    if not args.no_ensemble:
+        ens = ensemble_block(args.ensemble_sims, args.K)
+    srv = None
+    if not args.no_serve:
+        srv = serving_block(args.serve_requests, args.K)
+
     out = {
-        "metric": "Mcell-steps/s + %HBM roofline at 256^2x512 K=8",
+        "metric": "Mcell-steps/s + %HBM roofline at 256^2x512 K=8; ensemble sims/h, p50/p99 latency",
         "value": round(mcell, 2),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
@@ -231,6 +303,8 @@ def main() -> int:
             "dense_equiv_gbs": round(dense_equiv, 1),
         },
         "cpu_baseline": cpu,
+        "ensemble": ens,
+        "serving": srv,
     }
     ctx.close()
     print(json.dumps(out), flush=True)
