@@ -201,9 +201,11 @@ __device__ __forceinline__ void scan_point(const int* __restrict__ arg, int base
             g.ovf = true;
             continue;
         }
+        if (pos == g.n) {  // a new id takes the next free slot
 #pragma unroll
-        for (int q = 0; q < K; ++q) g.la[q] = (q == pos) ? id : g.la[q];
-        g.n += (pos == g.n) ? 1 : 0;
+            for (int q = 0; q < K; ++q) g.la[q] = (q == pos) ? id : g.la[q];
+            ++g.n;
+        }
         if (S >= 0) {
             const uint32_t sh = 4u * (uint32_t)pos;
             g.pm[S] = (g.pm[S] & ~(15u << sh)) | ((uint32_t)p << sh);
@@ -302,7 +304,7 @@ struct Rot {
 __device__ __forceinline__ Rot slot_rot(const StepArgs& a, int gid, float phC)
 {
     Rot r{1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
-    if ((double)phC > kLS) {
+    if (phC > kLSf) {
         const int tid = gid <= a.num_theta ? gid : (gid % a.num_theta) + 1;
         const float4* p = reinterpret_cast<const float4*>(a.R12 + 12 * tid);
         const float4 r0 = p[0], r1 = p[1], r2 = p[2];
@@ -401,7 +403,7 @@ __device__ __forceinline__ void rhs_point(const StepArgs& a, int i, int j, int k
         const float nv = slot_update(a, gid, v[0], v[1], v[2], v[3], v[4], v[5], v[6], repul, Up,
                                      a_s, epsv);
         phn[o] = nv;
-        agn[o] = ((double)nv < kKill) ? -1 : gid;
+        agn[o] = (nv < kKillf) ? -1 : gid;
         __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -450,6 +452,13 @@ __device__ __forceinline__ void slot_finish(const StepArgs& a, const Gather<K>& 
     if (g.n == 0) return;
     wr = !sp.skip;
     if (sp.skip) return;
+    // 0.25*(phiC_p+1)^2 of every filled slot, once per point
+    float rw[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        const float w = sp.pc[p] + 1.0f;
+        rw[p] = 0.25f * w * w;
+    }
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         if (q >= g.n) continue;
@@ -466,16 +475,12 @@ __device__ __forceinline__ void slot_finish(const StepArgs& a, const Gather<K>& 
         const int gid = g.la[q];
         float repul = 0.0f;
 #pragma unroll
-        for (int p = 0; p < K; ++p) {
-            if (p != q && p < g.n) {
-                const float w = sp.pc[p] + 1.0f;
-                repul += 0.25f * w * w;
-            }
-        }
+        for (int p = 0; p < K; ++p)
+            if (p != q && p < g.n) repul += rw[p];
         const float nv = slot_update_r(a, slot_rot(a, gid, sp.pc[q]), v[0], v[1], v[2], v[3], v[4], v[5], v[6], repul,
                                        sp.Up, a_s, epsv);
         out_ph[q] = nv;
-        out_id[q] = ((double)nv < kKill) ? -1 : gid;
+        out_id[q] = (nv < kKillf) ? -1 : gid;
     }
 }
 
@@ -776,9 +781,11 @@ __device__ __forceinline__ void scan_lds(const int* __restrict__ plane, int pt, 
             g.ovf = true;
             continue;
         }
+        if (pos == g.n) {  // a new id takes the next free slot
 #pragma unroll
-        for (int q = 0; q < K; ++q) g.la[q] = (q == pos) ? id : g.la[q];
-        g.n += (pos == g.n) ? 1 : 0;
+            for (int q = 0; q < K; ++q) g.la[q] = (q == pos) ? id : g.la[q];
+            ++g.n;
+        }
         if (S >= 0) {
             const uint32_t sh = 4u * (uint32_t)pos;
             g.pm[S] = (g.pm[S] & ~(15u << sh)) | ((uint32_t)p << sh);

@@ -12,6 +12,13 @@ namespace cupf {
 constexpr double kLS = -0.995;
 // Slot kill threshold (3DgrainMPI/APTPhaseField.cu:341), also a double compare.
 constexpr double kKill = -0.9999;
+// The same two comparisons in float, exact for every float x:
+// (double)x > -0.995 <=> x > -0.995f, because -0.995f is the largest float not
+// above -0.995; (double)x < -0.9999 <=> x < -0.9999f, because -0.9999f is the
+// smallest float not below -0.9999 (checked exhaustively on [-1, -0.99] by
+// tests/test_oracle.py).
+constexpr float kLSf = -0.995f;
+constexpr float kKillf = -0.9999f;
 
 // Leftover columns at the +x end narrower than this take the direct path;
 // wider ones form a partial LDS tile (its idle lanes cost less than the direct

@@ -550,3 +550,16 @@ def test_f32_build_tracks_f64_build():
     pb, ib = b.state()
     assert np.abs(pa.astype(np.float64) - pb).max() < 1e-3
     assert (ia != ib).mean() < 1e-3
+
+
+def test_float_thresholds_equal_double_compares():
+    """The step kernels compare in float against -0.995f and -0.9999f where the
+    reference compares (double)x against -0.995 and -0.9999: the same answer
+    for every float in [-1, -0.99] (all of them), and on both sides of it."""
+    lo, hi = np.float32(-1.0).view(np.uint32), np.float32(-0.99).view(np.uint32)
+    bits = np.arange(min(lo, hi), max(lo, hi) + 1, dtype=np.uint32)
+    x = bits.view(np.float32)
+    x = np.concatenate([x, np.float32([-2.0, -1.5, 0.0, 0.5, 1.0, -0.5])])
+    xd = x.astype(np.float64)
+    assert np.array_equal(xd > -0.995, x > np.float32(-0.995))
+    assert np.array_equal(xd < -0.9999, x < np.float32(-0.9999))
