@@ -233,6 +233,35 @@ def make_line_case(name: str, *, n: tuple, grains: int, K: int = 8, seed: int = 
     return Case(name, d, params, x, y, z, R9, phi0, alpha0, ng)
 
 
+def make_cone_case(name: str, *, n: tuple, K: int = 8, seed: int = 0, G: float = 5.0,
+                   R: float = 0.1, angle: float = 0.5, r0_frac: float = 0.35, z0: float = 2.0,
+                   V: float = 0.5, orientations: int = 1000) -> Case:
+    """All-liquid start under the cone melt-pool field (thermal type 3) on cone.py
+    physics (3DgrainMPI/cone.py:1-63: W0 = 0.15625 um, cfl 0.6), for nucleation.
+    The cone's radius r0 is r0_frac of the domain width; its tip starts at x = lm
+    and moves along x at V (in the units of the type-3 formula,
+    3DgrainMPI/APTPhaseField.cu:295-307).  Nucleated ids draw their orientation
+    from a pool of `orientations` seeded quaternions (id % NG + 1)."""
+    nx, ny, nz = n
+    inp = dict(CONE_PY)
+    dxd = float(inp["dx"]) * float(inp["W0"])
+    inp.update(Lx=float(nx) * dxd, asp_ratio_yx=ny / nx, asp_ratio_zx=nz / nx, moving_ratio=nz / nx,
+               G=float(G), R=float(R), UC=float(G) * float(R), z0=float(z0))
+    d = derive(inp, line_config=False)
+    if d["nx"] != nx or d["ny"] != ny or d["nz"] != nz:
+        raise ValueError(f"grid derivation gave {d['nx']}x{d['ny']}x{d['nz']}, wanted {n}")
+    d["r0"] = f32(r0_frac * float(d["lyd"]))
+    d["V"] = f32(V)
+    d["angle"] = f32(angle)
+    x, y, z = coords(d)
+    L = d["fnx"] * d["fny"] * d["fnz"]
+    phi0 = np.full(L, f32(-1.0), dtype=f32)
+    alpha0 = np.zeros(L, dtype=np.int32)
+    R9 = quaternion_rotations(orientations, seed)
+    params = kernel_params(d, K=K, num_theta=orientations, bc=(0, 0, 0), thermal_type=3, seed=seed)
+    return Case(name, d, params, x, y, z, R9, phi0, alpha0, 0)
+
+
 # Named workloads ------------------------------------------------------------------
 
 def c1(K: int = 8, **kw) -> Case:
@@ -245,3 +274,8 @@ def c2(K: int = 8, **kw) -> Case:
     kw.setdefault("line_config", True)
     kw.setdefault("nz_full", 1024)
     return make_line_case("line-256x256x512", n=(256, 256, 512), grains=5000, K=K, **kw)
+
+
+def c3(K: int = 8, **kw) -> Case:
+    """256^3 all-liquid start, cone melt-pool field moving along x, nucleation."""
+    return make_cone_case("cone-256^3", n=(256, 256, 256), K=K, **kw)
