@@ -167,6 +167,8 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--ensemble-sims", type=int, default=64)
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only run the batched-ensemble measurement and print its JSON object")
     ap.add_argument("--serve-requests", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -179,6 +181,10 @@ def main() -> int:
     if args.copy_only:
         gbs = capi.copy_bandwidth(args.copy_only, 5)
         print(json.dumps({"copy_bytes": args.copy_only, "copy_gbs": round(gbs, 1)}), flush=True)
+        return 0
+
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_block(args.ensemble_sims, args.K)}), flush=True)
         return 0
 
     t0 = time.time()
@@ -214,13 +220,14 @@ def main() -> int:
     # per-kernel durations (HIP events around every launch on the context's stream)
     nprof = min(args.steps, 100)
     proc0 = processed_points(ctx, p)
-    ms_main, ms_fix = ctx.step_profiled(nprof, step0)
+    ms_main, ms_fix, ms_plan = ctx.step_profiled(nprof, step0)
     step0 += nprof
     proc1 = processed_points(ctx, p)
     proc = (proc0 + proc1) // 2
     avg_main = ms_main / nprof
     act, tot = ctx.activity(0)
-    log(f"[bench] per launch: apt_step {avg_main:.4f} ms, apt_fixup {ms_fix / nprof:.4f} ms; "
+    log(f"[bench] per launch: apt_plan {ms_plan / nprof:.4f} ms, apt_step {avg_main:.4f} ms, "
+        f"fix-ups {ms_fix / nprof:.4f} ms; "
         f"processed points {proc0}..{proc1} of {n_upd} ({proc / n_upd:.1%}), "
         f"active tile planes {act}/{tot}")
 
@@ -296,6 +303,8 @@ def main() -> int:
             "copy_kernel_gbs": round(copy_gbs, 1),
             "frac_of_copy": round(achieved / copy_gbs, 4),
             "kernel_ms": round(avg_main, 4),
+            "plan_ms": round(ms_plan / nprof, 4),
+            "fixup_ms": round(ms_fix / nprof, 4),
             "algorithmic_bytes_per_launch": bytes_step,
             "processed_points_per_launch": proc,
             "updated_points_per_launch": n_upd,

@@ -102,7 +102,7 @@ def load() -> ctypes.CDLL:
         "pf_apply_bc": ([vp, c_int], c_int),
         "pf_step": ([vp, c_int, c_int, c_int], c_int),
         "pf_step_timed": ([vp, c_int, c_int, c_int, fp], c_int),
-        "pf_step_profiled": ([vp, c_int, c_int, c_int, fp, fp], c_int),
+        "pf_step_profiled": ([vp, c_int, c_int, c_int, fp, fp, fp], c_int),
         "pf_collect": ([vp, c_int, c_int, fp, ip], c_int),
         "pf_md_enable": ([vp, c_int], c_int),
         "pf_md_tip": ([vp, ip], c_int),
@@ -237,11 +237,12 @@ class APTContext:
         return ms.value
 
     def step_profiled(self, nsteps: int, first_step: int = 1, flags: int = 0):
-        """(ms in the main step kernel, ms in the fix-up kernel), summed over nsteps."""
-        m, f = ctypes.c_float(), ctypes.c_float()
+        """(ms in the main step kernel, ms in the fix-up kernels, ms in the
+        work-list plan kernel), each summed over nsteps."""
+        m, f, pl = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
         _check(load().pf_step_profiled(self._h, nsteps, first_step, flags, ctypes.byref(m),
-                                       ctypes.byref(f)))
-        return m.value, f.value
+                                       ctypes.byref(f), ctypes.byref(pl)))
+        return m.value, f.value, pl.value
 
     def sync(self):
         _check(load().pf_sync(self._h))

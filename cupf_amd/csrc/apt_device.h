@@ -515,50 +515,35 @@ template <int K>
 __device__ __forceinline__ void topk_push(TopK<K>& c, int id, float v, int pos)
 {
     int hit = -1;
+    float hv = 0.0f;
 #pragma unroll
-    for (int q = 0; q < K; ++q) hit = (q < c.n && c.id[q] == id) ? q : hit;
-    if (hit >= 0) {
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            if (q == hit && v > c.mx[q]) {
-                c.mx[q] = v;
-                c.pos[q] = pos;
-            }
-        }
-        return;
-    }
-    if (c.n < K) {
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            if (q == c.n) {
-                c.id[q] = id;
-                c.mx[q] = v;
-                c.pos[q] = pos;
-            }
-        }
-        ++c.n;
-        return;
+    for (int q = 0; q < K; ++q) {
+        const bool h = q < c.n && c.id[q] == id;
+        hit = h ? q : hit;
+        hv = h ? c.mx[q] : hv;
     }
     // worst candidate: smallest phi, latest position among equals
     int w = 0;
+    float wm = c.mx[0];
+    int wp = c.pos[0];
 #pragma unroll
     for (int q = 1; q < K; ++q) {
-        const bool worse = c.mx[q] < c.mx[w] || (c.mx[q] == c.mx[w] && c.pos[q] > c.pos[w]);
+        const bool worse = c.mx[q] < wm || (c.mx[q] == wm && c.pos[q] > wp);
         w = worse ? q : w;
+        wm = worse ? c.mx[q] : wm;
+        wp = worse ? c.pos[q] : wp;
     }
-    float wm = c.mx[0];
+    const bool full = c.n >= K;
+    const int tgt = hit >= 0 ? hit : (full ? w : c.n);
+    const bool upd = hit >= 0 ? (v > hv) : (!full || v > wm);
 #pragma unroll
-    for (int q = 1; q < K; ++q) wm = (q == w) ? c.mx[q] : wm;
-    if (v > wm) {
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            if (q == w) {
-                c.id[q] = id;
-                c.mx[q] = v;
-                c.pos[q] = pos;
-            }
-        }
+    for (int q = 0; q < K; ++q) {
+        const bool t = upd && q == tgt;
+        c.id[q] = t ? id : c.id[q];
+        c.mx[q] = t ? v : c.mx[q];
+        c.pos[q] = t ? pos : c.pos[q];
     }
+    c.n += (hit < 0 && !full) ? 1 : 0;
 }
 
 // Candidates of one neighbour point; P is its position in the scan order.

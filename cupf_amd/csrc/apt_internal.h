@@ -179,26 +179,27 @@ hipError_t launch_plan(const StepArgs& a, hipStream_t stream);
 hipError_t launch_batch_plan(const BatchArgs& b, hipStream_t stream);
 hipError_t launch_batch_advance(const BatchArgs& b, hipStream_t stream);
 hipError_t launch_step_k5_g7(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
-                             hipEvent_t e2);
+                             hipEvent_t e2, hipEvent_t e3);
 hipError_t launch_batch_k5_g7(const BatchArgs& b, hipStream_t stream);
 int step_slots_k5_g7(int device, bool batch);
 hipError_t launch_step_k5_g27(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
-                             hipEvent_t e2);
+                             hipEvent_t e2, hipEvent_t e3);
 hipError_t launch_batch_k5_g27(const BatchArgs& b, hipStream_t stream);
 int step_slots_k5_g27(int device, bool batch);
 hipError_t launch_step_k8_g7(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
-                             hipEvent_t e2);
+                             hipEvent_t e2, hipEvent_t e3);
 hipError_t launch_batch_k8_g7(const BatchArgs& b, hipStream_t stream);
 int step_slots_k8_g7(int device, bool batch);
 hipError_t launch_step_k8_g27(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
-                             hipEvent_t e2);
+                             hipEvent_t e2, hipEvent_t e3);
 hipError_t launch_batch_k8_g27(const BatchArgs& b, hipStream_t stream);
 int step_slots_k8_g27(int device, bool batch);
-// Main step kernel then the overflow fix-up kernel.  Optional events are
-// recorded before the main kernel, between the two and after the fix-up.
+// Work-list plan, main step kernel, then the overflow fix-up kernels.
+// Optional events are recorded before the plan, after it, after the main
+// kernel and after the fix-ups.
 hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,
-                       hipEvent_t ev_begin = nullptr, hipEvent_t ev_mid = nullptr,
-                       hipEvent_t ev_end = nullptr);
+                       hipEvent_t ev_begin = nullptr, hipEvent_t ev_planned = nullptr,
+                       hipEvent_t ev_mid = nullptr, hipEvent_t ev_end = nullptr);
 hipError_t launch_bc_fill(float* ph, int* arg, int fnx, int fny, int fnz, int K, int bcx,
                           int bcy, int bcz, hipStream_t stream);
 hipError_t launch_collect(const float* ph, const int* arg, float* phi, int* alpha, int L, int K,

@@ -448,12 +448,13 @@ hipError_t launch_batch_advance(const BatchArgs& b, hipStream_t stream)
 }
 
 hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,
-                       hipEvent_t ev_begin, hipEvent_t ev_mid, hipEvent_t ev_end)
+                       hipEvent_t ev_begin, hipEvent_t ev_planned, hipEvent_t ev_mid,
+                       hipEvent_t ev_end)
 {
-    if (K == 5) return gather27 ? launch_step_k5_g27(a, stream, ev_begin, ev_mid, ev_end)
-                                : launch_step_k5_g7(a, stream, ev_begin, ev_mid, ev_end);
-    if (K == 8) return gather27 ? launch_step_k8_g27(a, stream, ev_begin, ev_mid, ev_end)
-                                : launch_step_k8_g7(a, stream, ev_begin, ev_mid, ev_end);
+    if (K == 5) return gather27 ? launch_step_k5_g27(a, stream, ev_begin, ev_planned, ev_mid, ev_end)
+                                : launch_step_k5_g7(a, stream, ev_begin, ev_planned, ev_mid, ev_end);
+    if (K == 8) return gather27 ? launch_step_k8_g27(a, stream, ev_begin, ev_planned, ev_mid, ev_end)
+                                : launch_step_k8_g7(a, stream, ev_begin, ev_planned, ev_mid, ev_end);
     return hipErrorInvalidValue;
 }
 

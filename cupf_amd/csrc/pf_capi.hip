@@ -778,7 +778,7 @@ int pf_step_timed(pf_ctx* c, int nsteps, int first_step, int flags, float* ms)
 }
 
 int pf_step_profiled(pf_ctx* c, int nsteps, int first_step, int flags, float* ms_main,
-                     float* ms_fixup)
+                     float* ms_fixup, float* ms_plan)
 {
     int rc = ctx_check(c);
     if (rc) return rc;
@@ -786,7 +786,7 @@ int pf_step_profiled(pf_ctx* c, int nsteps, int first_step, int flags, float* ms
         return fail(PF_E_STATE, "pf_step needs coordinates, orientations and an initial state");
     if (flags & PF_STEP_NUCLEATE) return fail(PF_E_INVALID, "profiled stepping excludes nucleation");
     const pf_params& p = c->p;
-    std::vector<hipEvent_t> ev(3 * (size_t)nsteps, nullptr);
+    std::vector<hipEvent_t> ev(4 * (size_t)nsteps, nullptr);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     const int use_iuc = (flags & PF_STEP_INITIAL_UC) ? 1 : 0;
     hipError_t err = hipSuccess;
@@ -796,18 +796,21 @@ int pf_step_profiled(pf_ctx* c, int nsteps, int first_step, int flags, float* ms
         err = act_renew_if_needed(c);
         if (err != hipSuccess) break;
         StepArgs a = make_step_args(c, c->cur, t, use_iuc);
-        err = launch_step(a, p.K, p.gather27, c->stream, ev[3 * q], ev[3 * q + 1], ev[3 * q + 2]);
+        err = launch_step(a, p.K, p.gather27, c->stream, ev[4 * q], ev[4 * q + 1], ev[4 * q + 2],
+                          ev[4 * q + 3]);
         if (err != hipSuccess) break;
         ++c->nlaunch;
         c->ep[1 - c->cur] = a.ep_new;
         c->cur = 1 - c->cur;
     }
     if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
-    double tm = 0.0, tf = 0.0;
+    double tp = 0.0, tm = 0.0, tf = 0.0;
     for (int q = 0; q < nsteps && err == hipSuccess; ++q) {
-        float a1 = 0.0f, a2 = 0.0f;
-        err = hipEventElapsedTime(&a1, ev[3 * q], ev[3 * q + 1]);
-        if (err == hipSuccess) err = hipEventElapsedTime(&a2, ev[3 * q + 1], ev[3 * q + 2]);
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+        err = hipEventElapsedTime(&a0, ev[4 * q], ev[4 * q + 1]);
+        if (err == hipSuccess) err = hipEventElapsedTime(&a1, ev[4 * q + 1], ev[4 * q + 2]);
+        if (err == hipSuccess) err = hipEventElapsedTime(&a2, ev[4 * q + 2], ev[4 * q + 3]);
+        tp += a0;
         tm += a1;
         tf += a2;
     }
@@ -815,6 +818,7 @@ int pf_step_profiled(pf_ctx* c, int nsteps, int first_step, int flags, float* ms
     if (err != hipSuccess) return fail(PF_E_HIP, std::string("profiled step: ") + hipGetErrorString(err));
     if (ms_main) *ms_main = (float)tm;
     if (ms_fixup) *ms_fixup = (float)tf;
+    if (ms_plan) *ms_plan = (float)tp;
     return PF_OK;
 }
 

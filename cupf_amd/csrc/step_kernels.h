@@ -218,12 +218,13 @@ __global__ __launch_bounds__(kBlock) void batch_fixup(BatchArgs b)
 
 template <int K, bool G27>
 hipError_t launch_step_t(const StepArgs& a, hipStream_t stream, hipEvent_t ev_begin,
-                         hipEvent_t ev_mid, hipEvent_t ev_end)
+                         hipEvent_t ev_planned, hipEvent_t ev_mid, hipEvent_t ev_end)
 {
     dim3 grid(a.step_blocks), block(kBlock), fgrid(kFixupBlocks);
     if (ev_begin) (void)hipEventRecord(ev_begin, stream);
     hipError_t e = launch_plan(a, stream);
     if (e != hipSuccess) return e;
+    if (ev_planned) (void)hipEventRecord(ev_planned, stream);
     hipLaunchKernelGGL((apt_step_tiled<K, G27>), grid, block, 0, stream, a);
     if (ev_mid) (void)hipEventRecord(ev_mid, stream);
     hipLaunchKernelGGL((apt_fixup<K, G27>), fgrid, block, 0, stream, a);

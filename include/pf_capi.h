@@ -128,12 +128,13 @@ int pf_step(pf_ctx* ctx, int nsteps, int first_step, int flags);
 /* pf_step bracketed by HIP events on the context's stream; *ms = elapsed. */
 int pf_step_timed(pf_ctx* ctx, int nsteps, int first_step, int flags, float* ms);
 
-/* Diagnostic: pf_step with a HIP event pair around every launch of the main
- * step kernel and of the overflow fix-up kernel; the summed elapsed times
- * are returned in ms.  No nucleation.  No counterpart in the reference (its
+/* Diagnostic: pf_step with HIP events on the context's stream around every
+ * launch of the work-list plan kernel, of the main step kernel and of the
+ * overflow fix-up kernels; the summed elapsed times are returned in ms (any
+ * pointer may be NULL).  No nucleation.  No counterpart in the reference (its
  * only timer is the wall clock around the loop, 3DgrainMPI/APTPhaseField.cu:850,940). */
 int pf_step_profiled(pf_ctx* ctx, int nsteps, int first_step, int flags, float* ms_main,
-                     float* ms_fixup);
+                     float* ms_fixup, float* ms_plan);
 
 /* pf_collect flags */
 #define PF_COLLECT_CLEAR_ALPHA 1  /* zero alpha first (the cudaMemset at
