@@ -514,15 +514,20 @@ struct TopK {
 template <int K>
 __device__ __forceinline__ void topk_push(TopK<K>& c, int id, float v, int pos)
 {
-    int hit = -1;
-    float hv = 0.0f;
+    // kept id: raise its maximum (empty entries hold id -1, which never
+    // reaches here)
+    bool hit = false;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-        const bool h = q < c.n && c.id[q] == id;
-        hit = h ? q : hit;
-        hv = h ? c.mx[q] : hv;
+        const bool h = c.id[q] == id;
+        const bool u = h && v > c.mx[q];
+        c.mx[q] = u ? v : c.mx[q];
+        c.pos[q] = u ? pos : c.pos[q];
+        hit |= h;
     }
-    // worst candidate: smallest phi, latest position among equals
+    if (hit) return;
+    // new id: append while there is room, else replace the worst candidate
+    // (smallest phi, latest position among equals) if v beats it
     int w = 0;
     float wm = c.mx[0];
     int wp = c.pos[0];
@@ -534,8 +539,8 @@ __device__ __forceinline__ void topk_push(TopK<K>& c, int id, float v, int pos)
         wp = worse ? c.pos[q] : wp;
     }
     const bool full = c.n >= K;
-    const int tgt = hit >= 0 ? hit : (full ? w : c.n);
-    const bool upd = hit >= 0 ? (v > hv) : (!full || v > wm);
+    const int tgt = full ? w : c.n;
+    const bool upd = !full || v > wm;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         const bool t = upd && q == tgt;
@@ -543,7 +548,7 @@ __device__ __forceinline__ void topk_push(TopK<K>& c, int id, float v, int pos)
         c.mx[q] = t ? v : c.mx[q];
         c.pos[q] = t ? pos : c.pos[q];
     }
-    c.n += (hit < 0 && !full) ? 1 : 0;
+    c.n += full ? 0 : 1;
 }
 
 // Candidates of one neighbour point; P is its position in the scan order.
@@ -817,182 +822,6 @@ __device__ __forceinline__ void scan_all_lds(const int* pD, const int* pC, const
     }
 }
 
-// Overflow policy for one point from an LDS-staged neighbourhood (the rule of
-// apt_point_overflow: the K ids with the largest phi over their entries, ties
-// to the earliest entry reaching it, placed in first-appearance order): the
-// streaming top-K over the neighbourhood's slots, then pass A again over the
-// kept ids.  Used by the tiled fix-up kernel.
-template <int K>
-__device__ __forceinline__ void topk_lds(const int* ids, const float* phs, int off, int P, TopK<K>& c)
-{
-#pragma unroll
-    for (int p = 0; p < K; ++p) {
-        const int id = ids[p * kNPT + off];
-        if (id == -1) continue;
-        topk_push(c, id, phs[p * kNPT + off], P * K + p);
-    }
-}
-
-template <int K, int S>
-__device__ __forceinline__ void scan_lds_kept(const int* ids, int off, Gather<K>& g, const TopK<K>& kept)
-{
-#pragma unroll
-    for (int p = 0; p < K; ++p) {
-        const int id = ids[p * kNPT + off];
-        if (id == -1) continue;
-        bool ok = false;
-#pragma unroll
-        for (int u = 0; u < K; ++u) ok |= (kept.id[u] == id);
-        if (!ok) continue;
-        int pos = g.n;
-#pragma unroll
-        for (int q = 0; q < K; ++q) pos = (g.la[q] == id) ? q : pos;
-        if (pos == g.n) {
-#pragma unroll
-            for (int q = 0; q < K; ++q) g.la[q] = (q == pos) ? id : g.la[q];
-            ++g.n;
-        }
-        if (S >= 0) {
-            const uint32_t sh = 4u * (uint32_t)pos;
-            g.pm[S] = (g.pm[S] & ~(15u << sh)) | ((uint32_t)p << sh);
-        }
-    }
-}
-
-template <int K, bool G27>
-__device__ __forceinline__ void gather_overflow_lds(const int* iD, const int* iC, const int* iU,
-                                                    const float* pD, const float* pC,
-                                                    const float* pU, int pt, Gather<K>& g)
-{
-    TopK<K> c;
-    c.n = 0;
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-        c.id[q] = -1;
-        c.mx[q] = 0.0f;
-        c.pos[q] = 0;
-    }
-    gather_init(g);
-    if (!G27) {
-        topk_lds<K>(iD, pD, pt, 0, c);
-        topk_lds<K>(iC, pC, pt - kHX, 1, c);
-        topk_lds<K>(iC, pC, pt - 1, 2, c);
-        topk_lds<K>(iC, pC, pt, 3, c);
-        topk_lds<K>(iC, pC, pt + 1, 4, c);
-        topk_lds<K>(iC, pC, pt + kHX, 5, c);
-        topk_lds<K>(iU, pU, pt, 6, c);
-        scan_lds_kept<K, 0>(iD, pt, g, c);
-        scan_lds_kept<K, 1>(iC, pt - kHX, g, c);
-        scan_lds_kept<K, 2>(iC, pt - 1, g, c);
-        scan_lds_kept<K, 3>(iC, pt, g, c);
-        scan_lds_kept<K, 4>(iC, pt + 1, g, c);
-        scan_lds_kept<K, 5>(iC, pt + kHX, g, c);
-        scan_lds_kept<K, 6>(iU, pt, g, c);
-    } else {
-        // z-major 27-point order, position q = xi + 3*yi + 9*zi
-        auto off = [&](int q) { return pt + ((q / 3) % 3 - 1) * kHX + (q % 3 - 1); };
-#pragma unroll 1
-        for (int q = 0; q < 27; ++q) {
-            const int zi = q / 9;
-            topk_lds<K>(zi == 0 ? iD : (zi == 1 ? iC : iU), zi == 0 ? pD : (zi == 1 ? pC : pU), off(q),
-                        q, c);
-        }
-        auto run = [&](int from, int to) {
-#pragma unroll 1
-            for (int q = from; q < to; ++q) {
-                const int zi = q / 9;
-                scan_lds_kept<K, -1>(zi == 0 ? iD : (zi == 1 ? iC : iU), off(q), g, c);
-            }
-        };
-        run(0, 4);
-        scan_lds_kept<K, 0>(iD, pt, g, c);
-        run(5, 10);
-        scan_lds_kept<K, 1>(iC, pt - kHX, g, c);
-        run(11, 12);
-        scan_lds_kept<K, 2>(iC, pt - 1, g, c);
-        scan_lds_kept<K, 3>(iC, pt, g, c);
-        scan_lds_kept<K, 4>(iC, pt + 1, g, c);
-        run(15, 16);
-        scan_lds_kept<K, 5>(iC, pt + kHX, g, c);
-        run(17, 22);
-        scan_lds_kept<K, 6>(iU, pt, g, c);
-        run(23, 27);
-    }
-}
-
-// The tiled fix-up of one listed tile plane: stage planes k-1, k, k+1 of the
-// tile into the ring (as the step kernel does), then every lane marked in the
-// row masks computes its point under the overflow policy and stores it.  The
-// masks are cleared for the next step.  Ends with a barrier (ring reused).
-template <int K, bool G27>
-__device__ __forceinline__ void fixup_tile_plane(const StepArgs& a, int tp, int (*rid)[K * kNPT],
-                                                 float (*rph)[K * kNPT], unsigned long long* s_m)
-{
-    const int A = a.fnx * a.fny;
-    const int k = tp / a.ntiles;
-    const int tile = tp - k * a.ntiles;
-    const int ty = tile / a.ntx;
-    const int tx = tile - ty * a.ntx;
-    const int i0 = 1 + tx * kTX, j0 = 1 + ty * kTY;
-    if (threadIdx.x < 4) {
-        s_m[threadIdx.x] = a.ovf_mask[4 * tp + threadIdx.x];
-        a.ovf_mask[4 * tp + threadIdx.x] = 0ull;
-    }
-    PlaneLoader<K> ld;
-    loader_init(ld, a, i0, j0);
-    loader_fetch(ld, a, bc_map(k - 1, a.fnz, a.bcz) * A, true);
-    loader_store(ld, rid[0], rph[0]);
-    loader_fetch(ld, a, k * A, true);
-    loader_store(ld, rid[1], rph[1]);
-    loader_fetch(ld, a, bc_map(k + 1, a.fnz, a.bcz) * A, true);
-    loader_store(ld, rid[2], rph[2]);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int c = __popcll(s_m[0]) + __popcll(s_m[1]) + __popcll(s_m[2]) + __popcll(s_m[3]);
-        atomicAdd(a.overflow, (unsigned long long)c);
-    }
-    const int lx = threadIdx.x & (kTX - 1);
-    const int ly = threadIdx.x / kTX;
-    if ((s_m[ly] >> lx) & 1ull) {
-        const int i = i0 + lx, j = j0 + ly;
-        const int pt = (ly + 1) * kHX + (lx + 1);
-        const float a_s = 1.0f + 3.0f * a.kin_delta;
-        const float epsv = -4.0f * a.kin_delta / a_s;
-        const float* pD = rph[0];
-        const float* pC = rph[1];
-        const float* pU = rph[2];
-        Gather<K> g;
-        gather_overflow_lds<K, G27>(rid[0], rid[1], rid[2], pD, pC, pU, pt, g);
-        auto phi_at = [&](int s, uint32_t p) {
-            const int o = (int)p * kNPT;
-            switch (s) {
-            case 0: return pD[o + pt];
-            case 1: return pC[o + pt - kHX];
-            case 2: return pC[o + pt - 1];
-            case 3: return pC[o + pt];
-            case 4: return pC[o + pt + 1];
-            case 5: return pC[o + pt + kHX];
-            default: return pU[o + pt];
-            }
-        };
-        SlotPrep<K> sp;
-        slot_prep<K>(a, g, a.x[i], a.y[j], a.z[k], phi_at, sp);
-        int oid[K];
-        float oph[K];
-        bool wr;
-        slot_finish<K>(a, g, sp, a_s, epsv, phi_at, oid, oph, wr);
-        const int C = i + j * a.fnx + k * A;
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            if (wr || q >= g.n) {
-                a.arg_new[C + q * a.L] = oid[q];
-                a.ph_new[C + q * a.L] = oph[q];
-            }
-        }
-    }
-    __syncthreads();
-}
-
 // Append the overflowed lanes of a wave to the fix-up list (one atomic per wave).
 __device__ __forceinline__ void fixup_append(const StepArgs& a, int ovf, unsigned point)
 {
@@ -1071,22 +900,8 @@ __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld
     // A point that gathered an id may leave non-empty slots (an overflowed one
     // will, through the fix-up): mark the destination tile plane.
     if (ovf || g.n > 0) a.act_dst[k * a.ntiles + tile] = a.ep_new;
-    // Overflowed lanes are redone after the step: a row with many of them by
-    // the tiled fix-up kernel (mark them in the tile plane's row mask, list the
-    // tile plane once per step), a row with few by the per-point fix-up, which
-    // then costs less than staging three planes.
-    const unsigned long long om = __ballot(ovf);
-    const bool by_tile = __popcll(om) >= a.tile_fix_min;
-    fixup_append(a, ovf && !by_tile, (unsigned)C);
-    if (by_tile && (threadIdx.x & 63) == 0) {
-        const int tp = k * a.ntiles + tile;
-        atomicOr(&a.ovf_mask[4 * tp + (threadIdx.x >> 6)], om);
-        if (atomicExch(&a.ovf_claim[tp], a.ep_new) != a.ep_new) {
-            const unsigned q = atomicAdd(a.ovf_count, 1u);
-            a.ovf_items[2 * q] = (unsigned)a.sim_index;
-            a.ovf_items[2 * q + 1] = (unsigned)tp;
-        }
-    }
+    // Overflowed lanes are redone after the step by the per-point fix-up.
+    fixup_append(a, ovf, (unsigned)C);
     if (stage_next) {
         __syncthreads();
         loader_store(ld, rid[sD], rph[sD]);

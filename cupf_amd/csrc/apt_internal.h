@@ -44,18 +44,6 @@ struct StepArgs {
     // sets; cnt_next is the other one, cleared by this step's plan kernel.
     unsigned* fix_count;
     unsigned* cnt_next;
-    // Overflow on the LDS path, by tile plane: ovf_mask holds 4 words (one per
-    // row of the tile) of overflowed lanes per tile plane; ovf_claim the epoch
-    // of the step that last listed the tile plane; ovf_items the listed tile
-    // planes of this step as (simulation index, tile plane) pairs, ovf_count
-    // their number.  The tiled fix-up kernel redoes those lanes and clears the
-    // masks.
-    unsigned long long* ovf_mask;
-    unsigned* ovf_claim;
-    unsigned* ovf_items;
-    unsigned* ovf_count;
-    int sim_index;
-    int tile_fix_min;      // overflowed lanes in a tile row for the tiled fix-up (65: never)
     int* sink_i;           // 64-lane write sinks for lanes that must not store
     float* sink_f;
     // Activity map: one word per (column tile, z plane) and buffer.  A tile
@@ -114,7 +102,6 @@ struct BatchSim {
     int cur;               // source buffer of the next step
     int step;              // index s of the next step (t = (float)s*dt*tau0)
     int parity;            // counter set of the next step
-    int bparity;           // the batch's counter set of the next step (flips with every step)
     unsigned* cnt;         // the simulation's two 4-word counter sets
     float tau0;
     int plan_base;         // first plan lane of this simulation
@@ -126,8 +113,8 @@ constexpr int kMaxBatch = 1024;
 
 // A batched step over n simulations sharing K and the gather variant.  The
 // work list holds tile chunks of every simulation; cnt holds two 4-word
-// counter sets ([0] listed items, [1] next work index, [2] listed overflow
-// tile planes) used by alternate steps, selected by cnt[8].
+// counter sets ([0] listed items, [1] next work index) used by alternate
+// steps, selected by cnt[8].
 struct BatchArgs {
     BatchSim* sims;
     int n;
@@ -138,7 +125,6 @@ struct BatchArgs {
     unsigned long long* wl_need;
     unsigned long long* wl_act;
     unsigned* cnt;
-    unsigned* ovf_items;   // overflow tile planes of all simulations (see StepArgs)
     int step_blocks;
 };
 

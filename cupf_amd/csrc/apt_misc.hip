@@ -99,7 +99,6 @@ __global__ __launch_bounds__(64) void batch_advance(BatchArgs b)
         const unsigned p = b.cnt[8];
         b.cnt[4 * (1 - p)] = 0u;
         b.cnt[4 * (1 - p) + 1] = 0u;
-        b.cnt[4 * (1 - p) + 2] = 0u;
         b.cnt[8] = 1u - p;
     }
     if (s >= b.n) return;
@@ -127,8 +126,6 @@ __global__ __launch_bounds__(64) void batch_advance(BatchArgs b)
     a.fix_count = next;
     a.cnt_next = m.cnt + 4 * (1 - m.parity);
     a.wl_count = next + 2;
-    m.bparity = 1 - m.bparity;
-    a.ovf_count = b.cnt + 4 * m.bparity + 2;
     // t_cur_step = (2*kt+1)*dt*tau0 in float (3DgrainMPI/APTPhaseField.cu:863,919)
     a.t = (float)m.step * a.dt * m.tau0;
 }

@@ -65,10 +65,6 @@ struct pf_ctx {
     int tiles_x = 0, tiles_y = 0;   // LDS tiles (partial ones included)
     int step_blocks = 0;            // persistent step workgroups (occupancy x CUs)
     // overflow on the LDS path by tile plane (see StepArgs)
-    unsigned long long* ovf_mask = nullptr;
-    unsigned* ovf_claim = nullptr;
-    unsigned* ovf_items = nullptr;
-    int tile_fix_min = 32;          // see StepArgs::tile_fix_min (CUPF_TILEFIX)
     bool have_coords = false, have_orient = false, have_state = false;
 };
 
@@ -132,7 +128,6 @@ StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
         a.fix_count = c->fix_count + 4 * set;
         a.cnt_next = c->fix_count + 4 * (1 - set);
         a.wl_count = a.fix_count + 2;
-        a.ovf_count = a.fix_count + 1;
     }
     a.sink_i = c->sink;
     a.sink_f = reinterpret_cast<float*>(c->sink + 64);
@@ -162,11 +157,6 @@ StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
     a.wl_item = c->wl_item;
     a.wl_need = c->wl_need;
     a.wl_act = c->wl_act;
-    a.ovf_mask = c->ovf_mask;
-    a.ovf_claim = c->ovf_claim;
-    a.ovf_items = c->ovf_items;
-    a.sim_index = 0;
-    a.tile_fix_min = c->tile_fix_min;
 
     a.bcx = p.bcx;
     a.bcy = p.bcy;
@@ -289,7 +279,7 @@ void free_ctx(pf_ctx* c)
                     c->R12, c->psi_mac, c->T3d, c->phi0, c->phi, c->alpha, c->counters,
                     c->sums, c->alpha_full, c->loss, c->nstatus, c->ball,
                     c->fix_list, c->fix_count, c->sink, c->act[0], c->act[1],
-                    c->wl_item, c->wl_need, c->wl_act, c->ovf_mask, c->ovf_claim, c->ovf_items};
+                    c->wl_item, c->wl_need, c->wl_act};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -313,7 +303,6 @@ struct pf_batch {
     unsigned long long* wl_need = nullptr;
     unsigned long long* wl_act = nullptr;
     unsigned* cnt = nullptr;        // 2 x 4 list counters + parity word
-    unsigned* ovf_items = nullptr;  // overflow tile planes of all members (2 words each)
     size_t wl_cap = 0;
     // hipGraph of kGraphSteps batched steps, captured for one kchunk
     hipGraph_t graph = nullptr;
@@ -330,7 +319,7 @@ void batch_free(pf_batch* b)
     (void)hipSetDevice(b->device);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    void* bufs[] = {b->sims, b->wl_sim, b->wl_item, b->wl_need, b->wl_act, b->cnt, b->ovf_items};
+    void* bufs[] = {b->sims, b->wl_sim, b->wl_item, b->wl_need, b->wl_act, b->cnt};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -354,10 +343,6 @@ int batch_upload(pf_batch* b, const std::vector<int>& first_step, int use_iuc, B
         m.a.nzchunk = (p.fnz - 2 + b->kchunk - 1) / b->kchunk;
         m.a.edge_items = (int)(((long long)b->kchunk * c->edge_cols + 255) / 256);
         m.a.step_blocks = b->step_blocks;
-        m.a.ovf_items = b->ovf_items;
-        m.a.ovf_count = b->cnt + 2;  // the batch's counter set 0 (cleared below)
-        m.a.sim_index = i;
-        m.bparity = 0;
         for (int q = 0; q < 2; ++q) {
             m.ph[q] = c->ph[q];
             m.arg[q] = c->arg[q];
@@ -386,7 +371,6 @@ int batch_upload(pf_batch* b, const std::vector<int>& first_step, int use_iuc, B
     ba.wl_need = b->wl_need;
     ba.wl_act = b->wl_act;
     ba.cnt = b->cnt;
-    ba.ovf_items = b->ovf_items;
     ba.step_blocks = b->step_blocks;
     return PF_OK;
 }
@@ -488,10 +472,6 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         if (v >= 1 && v <= 61) c->kchunk = v;  // plane masks of a chunk fit 64 bits
     }
     if (const char* na = getenv("CUPF_NO_ACTIVITY")) c->use_act = atoi(na) ? 0 : 1;
-    if (const char* tf = getenv("CUPF_TILEFIX")) {
-        const int v = atoi(tf);
-        if (v >= 1 && v <= 65) c->tile_fix_min = v;
-    }
     c->ntx = (p.fnx - 2 + 63) / 64;
     c->nty = (p.fny - 2 + 3) / 4;
     c->ntiles = c->ntx * c->nty;
@@ -542,10 +522,7 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         (rc = dalloc(&c->act[1], (size_t)c->ntiles * p.fnz)) ||
         (rc = dalloc(&c->wl_item, (size_t)c->nzchunk * c->items_chunk + 1)) ||
         (rc = dalloc(&c->wl_need, (size_t)c->nzchunk * c->items_chunk + 1)) ||
-        (rc = dalloc(&c->wl_act, (size_t)c->nzchunk * c->items_chunk + 1)) ||
-        (rc = dalloc(&c->ovf_mask, (size_t)4 * c->ntiles * p.fnz)) ||
-        (rc = dalloc(&c->ovf_claim, (size_t)c->ntiles * p.fnz)) ||
-        (rc = dalloc(&c->ovf_items, (size_t)2 * c->ntiles * p.fnz)))
+        (rc = dalloc(&c->wl_act, (size_t)c->nzchunk * c->items_chunk + 1)))
         return bail(rc);
     c->h_sums.resize(p.fnz);
     for (int b = 0; b < 2; ++b) {
@@ -555,8 +532,6 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
     }
     if (hipMemsetAsync(c->counters, 0, 2 * sizeof(unsigned long long), c->stream) != hipSuccess ||
         hipMemsetAsync(c->fix_count, 0, 8 * sizeof(unsigned), c->stream) != hipSuccess ||
-        hipMemsetAsync(c->ovf_mask, 0, sizeof(unsigned long long) * 4 * c->ntiles * p.fnz, c->stream) != hipSuccess ||
-        hipMemsetAsync(c->ovf_claim, 0, sizeof(unsigned) * c->ntiles * p.fnz, c->stream) != hipSuccess ||
         hipMemsetAsync(c->alpha, 0, L * sizeof(int), c->stream) != hipSuccess ||
         hipMemsetAsync(c->act[0], 0, (size_t)c->ntiles * p.fnz * sizeof(unsigned), c->stream) != hipSuccess ||
         hipMemsetAsync(c->act[1], 0, (size_t)c->ntiles * p.fnz * sizeof(unsigned), c->stream) != hipSuccess ||
@@ -1079,10 +1054,7 @@ int pf_batch_create(pf_ctx* const* ctxs, int n, pf_batch** out)
     for (pf_ctx* c : b->ctxs)
         b->wl_cap += (size_t)((c->p.fnz - 2 + pf_batch::kMinChunk - 1) / pf_batch::kMinChunk) *
                      c->items_chunk;
-    size_t tp_cap = 0;
-    for (pf_ctx* c : b->ctxs) tp_cap += (size_t)c->ntiles * c->p.fnz;
     int rc;
-    if ((rc = dalloc(&b->ovf_items, 2 * tp_cap + 2))) return bail(rc);
     if ((rc = dalloc(&b->sims, (size_t)n)) || (rc = dalloc(&b->wl_sim, b->wl_cap + 1)) ||
         (rc = dalloc(&b->wl_item, b->wl_cap + 1)) || (rc = dalloc(&b->wl_need, b->wl_cap + 1)) ||
         (rc = dalloc(&b->wl_act, b->wl_cap + 1)) || (rc = dalloc(&b->cnt, 9)))

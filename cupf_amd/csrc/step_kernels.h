@@ -75,41 +75,6 @@ __global__ __launch_bounds__(kBlock) void apt_fixup(StepArgs a)
     }
 }
 
-// Tiled fix-up: the tile planes the step listed for overflowed lanes, one
-// workgroup at a time (the edge strip's overflowed points go to apt_fixup).
-template <int K, bool G27>
-__global__ __launch_bounds__(kBlock) void apt_fixup_tiles(StepArgs a)
-{
-    __shared__ int rid[3][K * kNPT];
-    __shared__ float rph[3][K * kNPT];
-    __shared__ unsigned long long s_m[4];
-    const unsigned n = __atomic_load_n(a.ovf_count, __ATOMIC_RELAXED);
-    for (unsigned q = blockIdx.x; q < n; q += gridDim.x)
-        fixup_tile_plane<K, G27>(a, (int)a.ovf_items[2 * q + 1], rid, rph, s_m);
-}
-
-// Batched tiled fix-up: every simulation's listed tile planes, each with its
-// simulation's step arguments staged in LDS.
-template <int K, bool G27>
-__global__ __launch_bounds__(kBlock) void batch_fixup_tiles(BatchArgs b)
-{
-    __shared__ int rid[3][K * kNPT];
-    __shared__ float rph[3][K * kNPT];
-    __shared__ unsigned long long s_m[4];
-    __shared__ StepArgs s_a;
-    constexpr int kWords = (int)(sizeof(StepArgs) / 4);
-    const unsigned* cnt = b.cnt + 4 * __atomic_load_n(b.cnt + 8, __ATOMIC_RELAXED);
-    const unsigned n = __atomic_load_n(cnt + 2, __ATOMIC_RELAXED);
-    for (unsigned q = blockIdx.x; q < n; q += gridDim.x) {
-        const int sim = (int)b.ovf_items[2 * q];
-        if ((int)threadIdx.x < kWords)
-            reinterpret_cast<unsigned*>(&s_a)[threadIdx.x] =
-                reinterpret_cast<const unsigned*>(&b.sims[sim].a)[threadIdx.x];
-        __syncthreads();
-        fixup_tile_plane<K, G27>(s_a, (int)b.ovf_items[2 * q + 1], rid, rph, s_m);
-    }
-}
-
 // Index of the simulation whose range [base_i, base_{i+1}) holds work index q
 // (bases ascending; sims with an empty range are skipped over).
 __device__ __forceinline__ int batch_find(const BatchSim* sims, int n, int q, bool edge)
@@ -228,7 +193,6 @@ hipError_t launch_step_t(const StepArgs& a, hipStream_t stream, hipEvent_t ev_be
     hipLaunchKernelGGL((apt_step_tiled<K, G27>), grid, block, 0, stream, a);
     if (ev_mid) (void)hipEventRecord(ev_mid, stream);
     hipLaunchKernelGGL((apt_fixup<K, G27>), fgrid, block, 0, stream, a);
-    hipLaunchKernelGGL((apt_fixup_tiles<K, G27>), fgrid, block, 0, stream, a);
     if (ev_end) (void)hipEventRecord(ev_end, stream);
     return hipGetLastError();
 }
@@ -241,7 +205,6 @@ hipError_t launch_batch_t(const BatchArgs& b, hipStream_t stream)
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((batch_step<K, G27>), grid, block, 0, stream, b);
     hipLaunchKernelGGL((batch_fixup<K, G27>), dim3(kFixupBlocks), block, 0, stream, b);
-    hipLaunchKernelGGL((batch_fixup_tiles<K, G27>), dim3(kFixupBlocks), block, 0, stream, b);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_batch_advance(b, stream);
