@@ -756,6 +756,41 @@ __device__ __forceinline__ void loader_store(const PlaneLoader<K>& ld, int* ids,
     }
 }
 
+// Stage a plane straight from global memory into an LDS ring slot (no
+// registers in between), for the start of a run when every slot of the ring
+// is free.  Element q * kBlock + threadIdx.x goes to the same LDS word as in
+// loader_store: the LDS address of a global_load_lds is the wave-uniform base
+// plus 4 * lane.  The caller waits for the loads (vmcnt) and then for the
+// other waves (barrier) before reading the slot.
+template <int K>
+__device__ __forceinline__ void loader_direct(const PlaneLoader<K>& ld, const StepArgs& a, int plane,
+                                              bool active, int* ids, float* phs)
+{
+    const int wbase = threadIdx.x & ~63;
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < PlaneLoader<K>::PER; ++q) {
+            if (q < PlaneLoader<K>::PER - 1 || ld.off[q] >= 0) {
+                const int o = ld.off[q] + plane;
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void*)(a.arg + o),
+                    (__attribute__((address_space(3))) void*)(ids + q * kBlock + wbase), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void*)(a.ph + o),
+                    (__attribute__((address_space(3))) void*)(phs + q * kBlock + wbase), 4, 0, 0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < PlaneLoader<K>::PER; ++q) {
+            if (q < PlaneLoader<K>::PER - 1 || ld.off[q] >= 0) {
+                ids[threadIdx.x + q * kBlock] = -1;
+                phs[threadIdx.x + q * kBlock] = -1.0f;
+            }
+        }
+    }
+}
+
 // Pass A from LDS for one neighbour point (same probe as scan_point).
 template <int K, int S>
 __device__ __forceinline__ void scan_lds(const int* __restrict__ plane, int pt, Gather<K>& g)
@@ -908,39 +943,78 @@ __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld
     }
 }
 
-// Planes of the chunk [k0, k1) of tile (tx, ty) to process, bit m for plane
-// k0+m, and the source activity of planes k0-1 .. k1, bit r for plane k0-1+r
-// (through the z boundary map): source words of the 3 x 3 tiles around the
-// tile at the mapped planes, destination words of the tile itself.
-__device__ __forceinline__ void chunk_masks(const StepArgs& a, int tx, int ty, int k0, int k1,
-                                            uint64_t& need, uint64_t& act)
+// The plan kernels' unit of work: wave gw of a simulation covers z-chunk
+// group cg = gw / (waves per group) and plan_tiles consecutive full tiles.
+// Lane l reads the activity of tile slot l / plan_rows at plane kb - 1 +
+// l % plan_rows (kb: first plane of the group): the 3 x 3 source words and
+// the tile's own destination word.  Two ballots then hold every chunk's bits,
+// and lane l = c * plan_tiles + slot assembles the masks of chunk c of the
+// group for that tile slot, chunk = [k0, k1): need bit m for plane k0+m to
+// process (some source word of the 3 x 3 tiles around the tile is active at a
+// plane within one of it, through the z map, or its own destination word is),
+// act bit r for the source activity of plane k0-1+r.  Items leave a wave
+// chunk-major, so the list runs through z-chunk groups in order and the items
+// in flight together lie in few planes.  put: the chunk has planes to
+// process; item = its work-list code.
+__device__ __forceinline__ void plan_wave(const StepArgs& a, int gw, int lane, bool& put, int& item,
+                                          uint64_t& need, uint64_t& act)
 {
-    const int n = k1 - k0;
-    const uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
-    if (!a.use_act) {
-        need = full;
-        act = ~0ull;
-        return;
+    const int R = plan_rows(a.kchunk);
+    const int TW = plan_tiles(a.kchunk);
+    const int C = plan_group(a.kchunk);
+    const int wpg = (a.items_chunk + TW - 1) / TW;  // waves per chunk group
+    const int cg = gw / wpg;
+    const int t0 = (gw - cg * wpg) * TW;
+    const int zc0 = cg * C;
+    const int kb = 1 + zc0 * a.kchunk;
+    const int klast = min(kb + C * a.kchunk, a.fnz - 1);  // upper halo plane of the last chunk
+    bool on = false, dst = false;
+    {
+        const int t = t0 + lane / R;
+        const int kk = kb - 1 + lane % R;
+        if (a.use_act && t < a.items_chunk && kk <= klast) {
+            const int ty = t / a.tiles_x;
+            const int tx = t - ty * a.tiles_x;
+            const int i0 = 1 + tx * kTX, j0 = 1 + ty * kTY;
+            const int xl = act_tile_x(bc_map(i0 - 1, a.fnx, a.bcx));
+            const int xr = act_tile_x(bc_map(min(i0 + kTX, a.fnx - 1), a.fnx, a.bcx));
+            const int yl = act_tile_y(bc_map(j0 - 1, a.fny, a.bcy));
+            const int yr = act_tile_y(bc_map(min(j0 + kTY, a.fny - 1), a.fny, a.bcy));
+            const int rl = yl * a.ntx, rc = ty * a.ntx, rr = yr * a.ntx;
+            const unsigned* w = a.act_src + bc_map(kk, a.fnz, a.bcz) * a.ntiles;
+            const unsigned e = a.ep_src;
+            on = w[xl + rl] >= e || w[tx + rl] >= e || w[xr + rl] >= e || w[xl + rc] >= e ||
+                 w[tx + rc] >= e || w[xr + rc] >= e || w[xl + rr] >= e || w[tx + rr] >= e ||
+                 w[xr + rr] >= e;
+            dst = kk >= kb && kk < klast && a.act_dst[kk * a.ntiles + tx + rc] >= a.ep_dst;
+        }
     }
-    const int i0 = 1 + tx * kTX, j0 = 1 + ty * kTY;
-    // halo columns/rows of the tile (a partial tile's end at the last interior point)
-    const int xl = act_tile_x(bc_map(i0 - 1, a.fnx, a.bcx));
-    const int xr = act_tile_x(bc_map(min(i0 + kTX, a.fnx - 1), a.fnx, a.bcx));
-    const int yl = act_tile_y(bc_map(j0 - 1, a.fny, a.bcy));
-    const int yr = act_tile_y(bc_map(min(j0 + kTY, a.fny - 1), a.fny, a.bcy));
-    const int rl = yl * a.ntx, rc = ty * a.ntx, rr = yr * a.ntx;
-    uint64_t ac = 0ull, dst = 0ull;
-    for (int r = 0; r < n + 2; ++r) {
-        const unsigned* w = a.act_src + bc_map(k0 - 1 + r, a.fnz, a.bcz) * a.ntiles;
-        const unsigned e = a.ep_src;
-        const bool on = w[xl + rl] >= e || w[tx + rl] >= e || w[xr + rl] >= e ||
-                        w[xl + rc] >= e || w[tx + rc] >= e || w[xr + rc] >= e ||
-                        w[xl + rr] >= e || w[tx + rr] >= e || w[xr + rr] >= e;
-        if (on) ac |= 1ull << r;
-        if (r < n && a.act_dst[(k0 + r) * a.ntiles + tx + rc] >= a.ep_dst) dst |= 1ull << r;
+    const uint64_t A = __ballot(on);
+    const uint64_t D = __ballot(dst);
+    const int c = lane / TW;
+    const int slot = lane - c * TW;
+    const int zc = zc0 + c;
+    const int t = t0 + slot;
+    put = false;
+    item = 0;
+    need = 0ull;
+    act = 0ull;
+    if (c < C && zc < a.nzchunk && t < a.items_chunk) {
+        const int k0 = 1 + zc * a.kchunk;
+        const int n = min(k0 + a.kchunk, a.fnz - 1) - k0;
+        const uint64_t full = (1ull << n) - 1ull;  // n <= 61
+        if (!a.use_act) {
+            need = full;
+            act = ~0ull;
+        } else {
+            const int sh = slot * R + c * a.kchunk;  // lane of plane k0 - 1 for this tile
+            const uint64_t ac = (A >> sh) & ((1ull << (n + 2)) - 1ull);
+            act = ac;
+            need = ((ac | (ac >> 1) | (ac >> 2)) | ((D >> (sh + 1)) & full)) & full;
+        }
+        put = need != 0ull;
+        item = zc * a.items_chunk + t;
     }
-    act = ac;
-    need = ((ac | (ac >> 1) | (ac >> 2)) | dst) & full;
 }
 
 template <int K, bool G27>
@@ -975,12 +1049,12 @@ __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, in
         // start of a run of processed planes: k-1, k, k+1 into the ring once
         // every wave is done with the ring
         __syncthreads();
-        loader_fetch(ld, a, bc_map(k - 1, a.fnz, a.bcz) * A, act_at(k - 1));
-        loader_store(ld, rid[(k + 2) % 3], rph[(k + 2) % 3]);
-        loader_fetch(ld, a, k * A, act_at(k));
-        loader_store(ld, rid[k % 3], rph[k % 3]);
-        loader_fetch(ld, a, bc_map(k + 1, a.fnz, a.bcz) * A, act_at(k + 1));
-        loader_store(ld, rid[(k + 1) % 3], rph[(k + 1) % 3]);
+        loader_direct(ld, a, bc_map(k - 1, a.fnz, a.bcz) * A, act_at(k - 1), rid[(k + 2) % 3],
+                      rph[(k + 2) % 3]);
+        loader_direct(ld, a, k * A, act_at(k), rid[k % 3], rph[k % 3]);
+        loader_direct(ld, a, bc_map(k + 1, a.fnz, a.bcz) * A, act_at(k + 1), rid[(k + 1) % 3],
+                      rph[(k + 1) % 3]);
+        __builtin_amdgcn_s_waitcnt(0);  // the three planes are in LDS (barrier in tile_plane)
         do {
             tile_plane<K, G27>(a, ld, rid, rph, zs, i, j, k, k0, tile, pt, lane_on, xi, yj, a_s,
                                epsv, need_at(k + 1), act_at(k + 2));
@@ -1032,7 +1106,7 @@ __device__ __forceinline__ void edge_point(const StepArgs& a, int p, int k0, int
     fixup_append(a, res == 2, (unsigned)(i + j * a.fnx + k * a.fnx * a.fny));
 }
 
-inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+__host__ __device__ inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
 

@@ -104,9 +104,22 @@ struct BatchSim {
     int parity;            // counter set of the next step
     unsigned* cnt;         // the simulation's two 4-word counter sets
     float tau0;
-    int plan_base;         // first plan lane of this simulation
+    int plan_base;         // first plan wave of this simulation
     int edge_base;         // first edge work index of this simulation
 };
+
+// Plan waves: a wave's 64 lanes cover plan_tiles tiles, plan_rows lanes (z
+// planes) each: a group of plan_group z chunks plus one halo plane at each
+// end (kchunk <= 61).
+__host__ __device__ inline int plan_rows(int kchunk) { return kchunk <= 14 ? 16 : (kchunk <= 30 ? 32 : 64); }
+__host__ __device__ inline int plan_tiles(int kchunk) { return 64 / plan_rows(kchunk); }
+__host__ __device__ inline int plan_group(int kchunk) { return (plan_rows(kchunk) - 2) / kchunk; }
+// Plan waves of a simulation with nzchunk chunks of items_chunk tiles.
+__host__ __device__ inline int plan_waves(int kchunk, int nzchunk, int items_chunk)
+{
+    const int g = plan_group(kchunk), tw = plan_tiles(kchunk);
+    return ((nzchunk + g - 1) / g) * ((items_chunk + tw - 1) / tw);
+}
 
 // Most simulations one batch may hold.
 constexpr int kMaxBatch = 1024;
@@ -118,7 +131,7 @@ constexpr int kMaxBatch = 1024;
 struct BatchArgs {
     BatchSim* sims;
     int n;
-    int plan_total;        // lanes of the plan kernel (all simulations' tile chunks)
+    int plan_total;        // waves of the plan kernel (all simulations)
     int edge_total;        // edge work items of all simulations
     int* wl_sim;
     int* wl_item;

@@ -7,74 +7,61 @@ namespace cupf {
 
 namespace {
 
-// Work list of one step (runs before apt_step_tiled): one lane per (z chunk,
-// full tile).  A tile chunk is listed only if some plane of it needs
-// processing, with its plane masks.  Edge items are not listed: the first
-// nzchunk * edge_items work indices are edge items, always taken (their
-// points decide one by one).
+// Work list of one step (runs before apt_step_tiled): waves over (z-chunk
+// group, tiles), see plan_wave.  A tile chunk is listed only if
+// some plane of it needs processing, with its plane masks.  Edge items are
+// not listed: the first nzchunk * edge_items work indices are edge items,
+// always taken (their points decide one by one).
 __global__ __launch_bounds__(kBlock) void apt_plan(StepArgs a)
 {
     const int g = blockIdx.x * kBlock + threadIdx.x;
     // the counter set of the next step (last used by the previous one, whose
     // fix-up has finished) starts from zero
     if (g < 4) a.cnt_next[g] = 0u;
-    const bool valid = g < a.nzchunk * a.items_chunk;
-    uint64_t need = 0ull, act = 0ull;
-    if (valid) {
-        const int zc = g / a.items_chunk;
-        const int t = g - zc * a.items_chunk;
-        const int ty = t / a.tiles_x;
-        const int tx = t - ty * a.tiles_x;
-        const int k0 = 1 + zc * a.kchunk;
-        const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-        chunk_masks(a, tx, ty, k0, k1, need, act);
-    }
-    const bool put = valid && need != 0ull;
+    const int gw = g >> 6;  // wave-uniform
+    if (gw >= plan_waves(a.kchunk, a.nzchunk, a.items_chunk)) return;
+    const int lane = threadIdx.x & 63;
+    bool put;
+    int item;
+    uint64_t need, act;
+    plan_wave(a, gw, lane, put, item, need, act);
     const unsigned long long m = __ballot(put);
     if (m == 0ull) return;
-    const int lane = threadIdx.x & 63;
     const int leader = __ffsll((long long)m) - 1;
     unsigned base = 0;
     if (lane == leader) base = atomicAdd(a.wl_count, (unsigned)__popcll(m));
     base = __shfl(base, leader);
     if (put) {
         const unsigned q = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-        a.wl_item[q] = g;
+        a.wl_item[q] = item;
         a.wl_need[q] = need;
         a.wl_act[q] = act;
     }
 }
 
-// Batched work list: one lane per (simulation, z chunk, tile).
+// Batched work list: every simulation's plan waves in turn; plan_base counts
+// waves.
 __global__ __launch_bounds__(kBlock) void batch_plan(BatchArgs b)
 {
-    const int g = blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = g < b.plan_total;
+    const int gw = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    if (gw >= b.plan_total) return;
     unsigned* cnt = b.cnt + 4 * __atomic_load_n(b.cnt + 8, __ATOMIC_RELAXED);
-    uint64_t need = 0ull, act = 0ull;
-    int sim = 0, gl = 0;
-    if (valid) {
-        int lo = 0, hi = b.n - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (b.sims[mid].plan_base <= g) lo = mid;
-            else hi = mid - 1;
-        }
-        sim = lo;
-        const StepArgs& a = b.sims[sim].a;
-        gl = g - b.sims[sim].plan_base;
-        const int zc = gl / a.items_chunk;
-        const int t = gl - zc * a.items_chunk;
-        const int ty = t / a.tiles_x;
-        const int tx = t - ty * a.tiles_x;
-        const int k0 = 1 + zc * a.kchunk;
-        const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-        chunk_masks(a, tx, ty, k0, k1, need, act);
+    int lo = 0, hi = b.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.sims[mid].plan_base <= gw) lo = mid;
+        else hi = mid - 1;
     }
-    const bool put = valid && need != 0ull;
+    const int sim = __builtin_amdgcn_readfirstlane(lo);
+    const StepArgs& a = b.sims[sim].a;
+    const int gl = gw - b.sims[sim].plan_base;
+    const int lane = threadIdx.x & 63;
+    bool put;
+    int item;
+    uint64_t need, act;
+    plan_wave(a, gl, lane, put, item, need, act);
     const unsigned long long m = __ballot(put);
     if (m == 0ull) return;
-    const int lane = threadIdx.x & 63;
     const int leader = __ffsll((long long)m) - 1;
     unsigned base = 0;
     if (lane == leader) base = atomicAdd(cnt, (unsigned)__popcll(m));
@@ -82,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void batch_plan(BatchArgs b)
     if (put) {
         const unsigned q = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
         b.wl_sim[q] = sim;
-        b.wl_item[q] = gl;
+        b.wl_item[q] = item;
         b.wl_need[q] = need;
         b.wl_act[q] = act;
     }
@@ -426,14 +413,14 @@ __global__ __launch_bounds__(kBlock) void copy_kernel(const float4* __restrict__
 
 hipError_t launch_plan(const StepArgs& a, hipStream_t stream)
 {
-    dim3 pgrid(max(1, cdiv((long long)a.nzchunk * a.items_chunk, kBlock)));
+    dim3 pgrid(max(1, cdiv(plan_waves(a.kchunk, a.nzchunk, a.items_chunk), kBlock / 64)));
     hipLaunchKernelGGL(apt_plan, pgrid, dim3(kBlock), 0, stream, a);
     return hipGetLastError();
 }
 
 hipError_t launch_batch_plan(const BatchArgs& b, hipStream_t stream)
 {
-    hipLaunchKernelGGL(batch_plan, dim3(max(1, cdiv(b.plan_total, kBlock))), dim3(kBlock), 0,
+    hipLaunchKernelGGL(batch_plan, dim3(max(1, cdiv(b.plan_total, kBlock / 64))), dim3(kBlock), 0,
                        stream, b);
     return hipGetLastError();
 }

@@ -357,7 +357,7 @@ int batch_upload(pf_batch* b, const std::vector<int>& first_step, int use_iuc, B
         m.tau0 = p.tau0;
         m.plan_base = plan_total;
         m.edge_base = edge_total;
-        plan_total += m.a.nzchunk * m.a.items_chunk;
+        plan_total += plan_waves(b->kchunk, m.a.nzchunk, m.a.items_chunk);
         edge_total += m.a.nzchunk * m.a.edge_items;
     }
     HIPCHK(hipMemcpyAsync(b->sims, h.data(), sizeof(BatchSim) * n, hipMemcpyHostToDevice, b->stream));
