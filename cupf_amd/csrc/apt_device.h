@@ -795,9 +795,14 @@ __device__ __forceinline__ void loader_direct(const PlaneLoader<K>& ld, const St
 template <int K, int S>
 __device__ __forceinline__ void scan_lds(const int* __restrict__ plane, int pt, Gather<K>& g)
 {
+    // all K reads first: a read inside the probe loop would sit behind the
+    // previous probe's branches and expose the LDS latency once per slot
+    int ids[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) ids[p] = plane[p * kNPT + pt];
 #pragma unroll
     for (int p = 0; p < K; ++p) {
-        const int id = plane[p * kNPT + pt];
+        const int id = ids[p];
         if (id == -1) continue;
         int pos = g.n;
 #pragma unroll
