@@ -791,15 +791,20 @@ __device__ __forceinline__ void loader_direct(const PlaneLoader<K>& ld, const St
     }
 }
 
-// Pass A from LDS for one neighbour point (same probe as scan_point).
-template <int K, int S>
-__device__ __forceinline__ void scan_lds(const int* __restrict__ plane, int pt, Gather<K>& g)
+// Pass A from LDS for one neighbour point (same probe as scan_point), in two
+// parts: the K id reads, then the probes.  Reads issued ahead of the probes
+// keep the LDS latency off the probe chain (a read inside the probe loop sits
+// behind the previous probe's branches and waits once per slot).
+template <int K>
+__device__ __forceinline__ void lds_ids(const int* __restrict__ plane, int pt, int (&ids)[K])
 {
-    // all K reads first: a read inside the probe loop would sit behind the
-    // previous probe's branches and expose the LDS latency once per slot
-    int ids[K];
 #pragma unroll
     for (int p = 0; p < K; ++p) ids[p] = plane[p * kNPT + pt];
+}
+
+template <int K, int S>
+__device__ __forceinline__ void probe_ids(const int (&ids)[K], Gather<K>& g)
+{
 #pragma unroll
     for (int p = 0; p < K; ++p) {
         const int id = ids[p];
@@ -821,6 +826,14 @@ __device__ __forceinline__ void scan_lds(const int* __restrict__ plane, int pt, 
             g.pm[S] = (g.pm[S] & ~(15u << sh)) | ((uint32_t)p << sh);
         }
     }
+}
+
+template <int K, int S>
+__device__ __forceinline__ void scan_lds(const int* __restrict__ plane, int pt, Gather<K>& g)
+{
+    int ids[K];
+    lds_ids<K>(plane, pt, ids);
+    probe_ids<K, S>(ids, g);
     __builtin_amdgcn_sched_barrier(0);
 }
 
@@ -829,13 +842,30 @@ __device__ __forceinline__ void scan_all_lds(const int* pD, const int* pC, const
                                              Gather<K>& g)
 {
     if (!G27) {
-        scan_lds<K, 0>(pD, pt, g);
-        scan_lds<K, 1>(pC, pt - kHX, g);
-        scan_lds<K, 2>(pC, pt - 1, g);
-        scan_lds<K, 3>(pC, pt, g);
-        scan_lds<K, 4>(pC, pt + 1, g);
-        scan_lds<K, 5>(pC, pt + kHX, g);
-        scan_lds<K, 6>(pU, pt, g);
+        // two id buffers: the next neighbour's reads are in flight while the
+        // current one is probed
+        int b0[K], b1[K];
+        lds_ids<K>(pD, pt, b0);
+        lds_ids<K>(pC, pt - kHX, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        probe_ids<K, 0>(b0, g);
+        lds_ids<K>(pC, pt - 1, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        probe_ids<K, 1>(b1, g);
+        lds_ids<K>(pC, pt, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        probe_ids<K, 2>(b0, g);
+        lds_ids<K>(pC, pt + 1, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        probe_ids<K, 3>(b1, g);
+        lds_ids<K>(pC, pt + kHX, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        probe_ids<K, 4>(b0, g);
+        lds_ids<K>(pU, pt, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        probe_ids<K, 5>(b1, g);
+        probe_ids<K, 6>(b0, g);
+        __builtin_amdgcn_sched_barrier(0);
     } else {
         // z-major 27-point order as in scan_all; non-stencil positions rolled
         auto run = [&](int from, int to) {
