@@ -1102,41 +1102,57 @@ int pf_batch_step(pf_batch* b, int nsteps, const int* first_step, int flags)
     const bool use_graph = (flags & PF_BATCH_GRAPH) != 0;
     std::vector<int> fs(first_step, first_step + b->ctxs.size());
     int left = nsteps;
-    if ((flags & PF_BATCH_AUTOTUNE) && nsteps >= 64) {
-        // the chunk depth changes only how the work is cut, never the results:
-        // time a few real steps with each candidate and keep the fastest
-        const int cand[] = {2, 4, 8, 16};
-        const int m = std::min(16, nsteps / 8);
-        float best = 0.0f;
-        int best_k = b->kchunk;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        int rc = PF_OK;
-        for (int kc : cand) {
-            b->kchunk = kc;
-            if ((rc = batch_run(b, 1, fs, use_iuc, false))) break;  // warm the chunk depth
-            left -= 1;
-            if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(PF_E_HIP, "event"); break; }
-            if ((rc = batch_run(b, m, fs, use_iuc, false))) break;
-            left -= m;
-            float ms = 0.0f;
-            if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-                rc = fail(PF_E_HIP, "autotune timing");
-                break;
+    if (!(flags & PF_BATCH_AUTOTUNE)) return batch_run(b, left, fs, use_iuc, use_graph);
+    // The chunk depth changes only how the work is cut, never the results.
+    // Every kTuneEvery steps, time a few real steps with each candidate and
+    // keep the fastest for the rest of the segment: the best depth moves as
+    // the simulations evolve (overflow start-up, then thin fronts).  The
+    // candidates are timed in two rounds, the second in reverse order, so a
+    // cost that drifts steadily over the tuning steps weighs on all alike.
+    constexpr int kTuneEvery = 400, kTuneSteps = 4;
+    const int cand[] = {2, 4, 8, 16};
+    constexpr int nc = (int)(sizeof(cand) / sizeof(cand[0]));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = PF_OK;
+    while (left > 0 && rc == PF_OK) {
+        if (left >= 2 * nc * (kTuneSteps + 1) + 16) {
+            float tot[nc] = {};
+            for (int round = 0; round < 2 && rc == PF_OK; ++round) {
+                for (int u = 0; u < nc && rc == PF_OK; ++u) {
+                    const int c = round == 0 ? u : nc - 1 - u;
+                    b->kchunk = cand[c];
+                    if ((rc = batch_run(b, 1, fs, use_iuc, false))) break;  // warm the depth
+                    left -= 1;
+                    if (hipEventRecord(e0, b->stream) != hipSuccess) {
+                        rc = fail(PF_E_HIP, "event");
+                        break;
+                    }
+                    if ((rc = batch_run(b, kTuneSteps, fs, use_iuc, false))) break;
+                    left -= kTuneSteps;
+                    float ms = 0.0f;
+                    if (hipEventRecord(e1, b->stream) != hipSuccess ||
+                        hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                        rc = fail(PF_E_HIP, "autotune timing");
+                        break;
+                    }
+                    tot[c] += ms;
+                }
             }
-            if (best == 0.0f || ms < best) {
-                best = ms;
-                best_k = kc;
-            }
+            if (rc) break;
+            int best = 0;
+            for (int c = 1; c < nc; ++c) best = tot[c] < tot[best] ? c : best;
+            b->kchunk = cand[best];
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (rc) return rc;
-        b->kchunk = best_k;
+        const int seg = std::min(left, kTuneEvery);
+        rc = batch_run(b, seg, fs, use_iuc, use_graph);
+        left -= seg;
     }
-    return batch_run(b, left, fs, use_iuc, use_graph);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
 }
 
 int pf_sync(pf_ctx* c)

@@ -168,11 +168,12 @@ class Server:
     """
 
     def __init__(self, hbm_budget: int = 64 << 30, quantum: int = 200, K: int = 8,
-                 graph: bool = True):
+                 graph: bool = True, autotune: bool = True):
         self.budget = int(hbm_budget)
         self.quantum = int(quantum)
         self.K = K
         self.graph = graph
+        self.autotune = autotune
 
     def run(self, specs, prepared, clock=time.perf_counter):
         pending = sorted(range(len(specs)), key=lambda q: specs[q].arrival)
@@ -181,6 +182,7 @@ class Server:
         used = 0
         t0 = clock()
         batch, batch_ids = None, None
+        kchunk = None  # the last tuned chunk depth, carried into re-formed batches
         while pending or running:
             now = clock() - t0
             admitted = False
@@ -206,10 +208,16 @@ class Server:
                     batch.close()
                 batch = capi.Batch([r[1] for r in running], [r[2] for r in running])
                 batch_ids = ids
+                if kchunk is not None:
+                    batch.kchunk = kchunk
             left = [specs[r[0]].nsteps - r[2] + 1 for r in running]
             n = min(self.quantum, min(left))
             batch.next_step = [r[2] for r in running]
-            batch.step(n, capi.PF_BATCH_GRAPH if (self.graph and n >= 8) else 0)
+            flags = capi.PF_BATCH_GRAPH if (self.graph and n >= 8) else 0
+            if self.autotune and n >= 128:
+                flags |= capi.PF_BATCH_AUTOTUNE
+            batch.step(n, flags)
+            kchunk = batch.kchunk
             for r in running:
                 r[2] += n
             still = []

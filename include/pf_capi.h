@@ -203,8 +203,10 @@ typedef struct pf_batch pf_batch;
 
 /* pf_batch_step flags (beside PF_STEP_INITIAL_UC) */
 #define PF_BATCH_GRAPH 4      /* replay groups of 8 steps from one captured hipGraph */
-#define PF_BATCH_AUTOTUNE 8   /* time a few real steps with z-chunk depths 2, 4, 8, 16
-                                 and keep the fastest (needs nsteps >= 64) */
+#define PF_BATCH_AUTOTUNE 8   /* every 400 steps, time 4 real steps with each z-chunk
+                                 depth 2, 4, 8, 16, twice (forward, then reverse
+                                 order), and keep the fastest for the next 400 steps
+                                 (a tuning round needs >= 56 steps left) */
 
 /* n <= 1024 distinct contexts. */
 int pf_batch_create(pf_ctx* const* ctxs, int n, pf_batch** out);
