@@ -29,7 +29,8 @@ __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
         // the tail of the list balanced
         if (threadIdx.x == 0) s_q = atomicAdd(a.wl_count + 1, 1u);
         __syncthreads();
-        const unsigned q = s_q;
+        // uniform over the workgroup: the item's fields come through scalar loads
+        const unsigned q = __builtin_amdgcn_readfirstlane(s_q);
         __syncthreads();
         if (q >= n) break;
         if (q < ne) {
@@ -96,32 +97,28 @@ template <int K, bool G27>
 __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
 {
     __shared__ unsigned s_q;
-    // the current item's step arguments, staged once per item: the code reads
-    // its fields from LDS (uniform addresses) instead of every lane copying the
-    // whole block from global memory into registers
-    __shared__ StepArgs s_a;
-    constexpr int kWords = (int)(sizeof(StepArgs) / 4);
-    static_assert(sizeof(StepArgs) % 4 == 0 && kWords <= kBlock, "StepArgs staging");
+    // The members' records do not change during the kernel (batch_advance
+    // rewrites them after it): read through the constant address space with a
+    // workgroup-uniform index, every step-argument field is a scalar load.
+    typedef const __attribute__((address_space(4))) BatchSim CSim;
+    const CSim* sims = (const CSim*)b.sims;
     const unsigned* cnt = b.cnt + 4 * __atomic_load_n(b.cnt + 8, __ATOMIC_RELAXED);
     const unsigned ne = (unsigned)b.edge_total;
     const unsigned n = ne + __atomic_load_n(cnt, __ATOMIC_RELAXED);
     for (;;) {
         if (threadIdx.x == 0) s_q = atomicAdd((unsigned*)cnt + 1, 1u);
         __syncthreads();
-        const unsigned q = s_q;
+        const unsigned q = __builtin_amdgcn_readfirstlane(s_q);
         if (q >= n) break;
         const bool edge = q < ne;
         const unsigned w = q - ne;
-        const int sim = edge ? batch_find(b.sims, b.n, (int)q, true) : b.wl_sim[w];
-        if ((int)threadIdx.x < kWords)
-            reinterpret_cast<unsigned*>(&s_a)[threadIdx.x] =
-                reinterpret_cast<const unsigned*>(&b.sims[sim].a)[threadIdx.x];
-        __syncthreads();
-        const StepArgs& a = s_a;
+        const int sim =
+            __builtin_amdgcn_readfirstlane(edge ? batch_find(b.sims, b.n, (int)q, true) : b.wl_sim[w]);
+        const StepArgs& a = *(const StepArgs*)&sims[sim].a;
         const float a_s = 1.0f + 3.0f * a.kin_delta;
         const float epsv = -4.0f * a.kin_delta / a_s;
         if (edge) {
-            const int e0 = (int)q - b.sims[sim].edge_base;
+            const int e0 = (int)q - sims[sim].edge_base;
             const int zc = e0 / a.edge_items;
             const int e = e0 - zc * a.edge_items;
             const int k0 = 1 + zc * a.kchunk;
@@ -137,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
             tile_march<K, G27>(a, tx, ty, k0, k1, b.wl_need[w], b.wl_act[w], a_s, epsv);
         }
-        __syncthreads();  // s_q and s_a are rewritten for the next item
+        __syncthreads();  // s_q is rewritten for the next item
     }
 }
 
