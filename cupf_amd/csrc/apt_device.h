@@ -190,9 +190,15 @@ template <int K, int S>
 __device__ __forceinline__ void scan_point(const int* __restrict__ arg, int base, int L,
                                            Gather<K>& g)
 {
+    // the K loads go out together: a load inside the probe loop would sit
+    // behind the previous probe's branches and wait out the memory latency
+    // once per slot
+    int ids[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) ids[p] = arg[base + p * L];
 #pragma unroll
     for (int p = 0; p < K; ++p) {
-        const int id = arg[base + p * L];
+        const int id = ids[p];
         if (id == -1) continue;
         int pos = g.n;
 #pragma unroll
@@ -211,8 +217,8 @@ __device__ __forceinline__ void scan_point(const int* __restrict__ arg, int base
             g.pm[S] = (g.pm[S] & ~(15u << sh)) | ((uint32_t)p << sh);
         }
     }
-    // Keep one neighbour point's loads in flight per lane; occupancy supplies
-    // the memory parallelism instead of registers.
+    // one neighbour point's loads in flight per lane; occupancy supplies the
+    // rest of the memory parallelism
     __builtin_amdgcn_sched_barrier(0);
 }
 
@@ -365,12 +371,16 @@ __device__ __forceinline__ void rhs_point(const StepArgs& a, int i, int j, int k
     float T;
     const bool skip = thermal_T(a, i, j, k, T);
     const float Up = T / a.L_cp;
-    // centre phi of every slot first: the repulsion sums over the other slots
+    // centre phi of every slot first: the repulsion sums over the other slots.
+    // Reads are unconditional (an absent slot reads slot 0 and is replaced by
+    // -1 afterwards) so each group of them goes out together instead of one
+    // memory latency per read.
     float pc[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         const uint32_t p = (g.pm[3] >> (4 * q)) & 15u;
-        pc[q] = (q < g.n && p != 15u) ? phi_at(3, p) : -1.0f;
+        const float x = phi_at(3, p < (uint32_t)K ? p : 0u);
+        pc[q] = (q < g.n && p != 15u) ? x : -1.0f;
     }
 #pragma unroll
     for (int q = 0; q < K; ++q) {
@@ -388,7 +398,8 @@ __device__ __forceinline__ void rhs_point(const StepArgs& a, int i, int j, int k
                 v[s] = pc[q];
             } else {
                 const uint32_t p = (g.pm[s] >> (4 * q)) & 15u;
-                v[s] = (p != 15u) ? phi_at(s, p) : -1.0f;
+                const float x = phi_at(s, p < (uint32_t)K ? p : 0u);
+                v[s] = (p != 15u) ? x : -1.0f;
             }
         }
         const int gid = g.la[q];
@@ -555,11 +566,17 @@ __device__ __forceinline__ void topk_push(TopK<K>& c, int id, float v, int pos)
 template <int K>
 __device__ __forceinline__ void topk_point(const StepArgs& a, int base, int P, TopK<K>& c)
 {
+    int ids[K];  // all 2K loads first, as in scan_point
+    float phs[K];
 #pragma unroll
     for (int p = 0; p < K; ++p) {
-        const int id = a.arg[base + p * a.L];
-        if (id == -1) continue;
-        topk_push(c, id, a.ph[base + p * a.L], P * K + p);
+        ids[p] = a.arg[base + p * a.L];
+        phs[p] = a.ph[base + p * a.L];
+    }
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        if (ids[p] == -1) continue;
+        topk_push(c, ids[p], phs[p], P * K + p);
     }
     __builtin_amdgcn_sched_barrier(0);
 }
@@ -569,9 +586,12 @@ template <int K, int S>
 __device__ __forceinline__ void scan_point_kept(const int* __restrict__ arg, int base, int L,
                                                 Gather<K>& g, const TopK<K>& kept)
 {
+    int ids[K];  // the K loads first, as in scan_point
+#pragma unroll
+    for (int p = 0; p < K; ++p) ids[p] = arg[base + p * L];
 #pragma unroll
     for (int p = 0; p < K; ++p) {
-        const int id = arg[base + p * L];
+        const int id = ids[p];
         if (id == -1) continue;
         bool ok = false;
 #pragma unroll

@@ -66,6 +66,7 @@ __global__ __launch_bounds__(kBlock) void apt_fixup(StepArgs a)
     const float a_s = 1.0f + 3.0f * a.kin_delta;
     const float epsv = -4.0f * a.kin_delta / a_s;
     const int A = a.fnx * a.fny;
+#pragma unroll 1
     for (unsigned q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
         const int C = (int)a.fix_list[q];
         const int k = C / A;
