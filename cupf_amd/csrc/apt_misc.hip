@@ -7,32 +7,57 @@ namespace cupf {
 
 namespace {
 
+// A plan workgroup appends all its listed chunks with one atomic (the list
+// counter is a single word every workgroup hits): 4 waves for one simulation,
+// 16 for a batch, whose plan has many more waves.
+constexpr int kBatchPlanBlock = 1024;
+
+// List position of this lane's chunk if `put`; every thread of the workgroup
+// (kPlanWaves waves) calls it.  Waves take consecutive runs of positions in
+// wave order.
+template <int kPlanWaves>
+__device__ __forceinline__ unsigned plan_slot(bool put, unsigned* counter, unsigned* s_cnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(put);
+    if (lane == 0) s_cnt[w] = (unsigned)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned tot = 0u;
+        for (int u = 0; u < kPlanWaves; ++u) {
+            const unsigned c = s_cnt[u];
+            s_cnt[u] = tot;
+            tot += c;
+        }
+        s_cnt[kPlanWaves] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    return s_cnt[kPlanWaves] + s_cnt[w] + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+}
+
 // Work list of one step (runs before apt_step_tiled): waves over (z-chunk
-// group, tiles), see plan_wave.  A tile chunk is listed only if
-// some plane of it needs processing, with its plane masks.  Edge items are
-// not listed: the first nzchunk * edge_items work indices are edge items,
-// always taken (their points decide one by one).
+// group, tiles), see plan_wave.  A tile chunk is listed only if some plane of
+// it needs processing, with its plane masks.  Edge items are not listed: the
+// first nzchunk * edge_items work indices are edge items, always taken (their
+// points decide one by one).
 __global__ __launch_bounds__(kBlock) void apt_plan(StepArgs a)
 {
+    constexpr int kPlanWaves = kBlock / 64;
+    __shared__ unsigned s_cnt[kPlanWaves + 1];
     const int g = blockIdx.x * kBlock + threadIdx.x;
     // the counter set of the next step (last used by the previous one, whose
     // fix-up has finished) starts from zero
     if (g < 4) a.cnt_next[g] = 0u;
     const int gw = g >> 6;  // wave-uniform
-    if (gw >= plan_waves(a.kchunk, a.nzchunk, a.items_chunk)) return;
     const int lane = threadIdx.x & 63;
-    bool put;
-    int item;
-    uint64_t need, act;
-    plan_wave(a, gw, lane, put, item, need, act);
-    const unsigned long long m = __ballot(put);
-    if (m == 0ull) return;
-    const int leader = __ffsll((long long)m) - 1;
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(a.wl_count, (unsigned)__popcll(m));
-    base = __shfl(base, leader);
+    bool put = false;
+    int item = 0;
+    uint64_t need = 0ull, act = 0ull;
+    if (gw < plan_waves(a.kchunk, a.nzchunk, a.items_chunk))
+        plan_wave(a, gw, lane, put, item, need, act);
+    const unsigned q = plan_slot<kPlanWaves>(put, a.wl_count, s_cnt);
     if (put) {
-        const unsigned q = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
         a.wl_item[q] = item;
         a.wl_need[q] = need;
         a.wl_act[q] = act;
@@ -41,33 +66,34 @@ __global__ __launch_bounds__(kBlock) void apt_plan(StepArgs a)
 
 // Batched work list: every simulation's plan waves in turn; plan_base counts
 // waves.
-__global__ __launch_bounds__(kBlock) void batch_plan(BatchArgs b)
+__global__ __launch_bounds__(kBatchPlanBlock) void batch_plan(BatchArgs b)
 {
-    const int gw = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-    if (gw >= b.plan_total) return;
+    constexpr int kPlanWaves = kBatchPlanBlock / 64;
+    __shared__ unsigned s_cnt[kPlanWaves + 1];
+    // wave-uniform index and member records read through the constant address
+    // space (batch_advance rewrites them only after this kernel): the search
+    // and the step-argument fields are scalar loads
+    const int gw = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBatchPlanBlock + threadIdx.x) >> 6));
+    typedef const __attribute__((address_space(4))) BatchSim CSim;
+    const CSim* sims = (const CSim*)b.sims;
     unsigned* cnt = b.cnt + 4 * __atomic_load_n(b.cnt + 8, __ATOMIC_RELAXED);
-    int lo = 0, hi = b.n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (b.sims[mid].plan_base <= gw) lo = mid;
-        else hi = mid - 1;
-    }
-    const int sim = __builtin_amdgcn_readfirstlane(lo);
-    const StepArgs& a = b.sims[sim].a;
-    const int gl = gw - b.sims[sim].plan_base;
     const int lane = threadIdx.x & 63;
-    bool put;
-    int item;
-    uint64_t need, act;
-    plan_wave(a, gl, lane, put, item, need, act);
-    const unsigned long long m = __ballot(put);
-    if (m == 0ull) return;
-    const int leader = __ffsll((long long)m) - 1;
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(cnt, (unsigned)__popcll(m));
-    base = __shfl(base, leader);
+    bool put = false;
+    int item = 0, sim = 0;
+    uint64_t need = 0ull, act = 0ull;
+    if (gw < b.plan_total) {
+        int lo = 0, hi = b.n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sims[mid].plan_base <= gw) lo = mid;
+            else hi = mid - 1;
+        }
+        sim = lo;
+        const StepArgs& a = *(const StepArgs*)&sims[sim].a;
+        plan_wave(a, gw - sims[sim].plan_base, lane, put, item, need, act);
+    }
+    const unsigned q = plan_slot<kPlanWaves>(put, cnt, s_cnt);
     if (put) {
-        const unsigned q = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
         b.wl_sim[q] = sim;
         b.wl_item[q] = item;
         b.wl_need[q] = need;
@@ -420,7 +446,8 @@ hipError_t launch_plan(const StepArgs& a, hipStream_t stream)
 
 hipError_t launch_batch_plan(const BatchArgs& b, hipStream_t stream)
 {
-    hipLaunchKernelGGL(batch_plan, dim3(max(1, cdiv(b.plan_total, kBlock / 64))), dim3(kBlock), 0,
+    hipLaunchKernelGGL(batch_plan, dim3(max(1, cdiv(b.plan_total, kBatchPlanBlock / 64))),
+                       dim3(kBatchPlanBlock), 0,
                        stream, b);
     return hipGetLastError();
 }
