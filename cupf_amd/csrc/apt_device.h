@@ -241,15 +241,23 @@ __device__ __forceinline__ int nbase(const StepArgs& a, const Nbr& nb, int xi, i
     return nb.xs[xi] + nb.ys[yi] * a.fnx + nb.zs[zi] * a.fnx * a.fny;
 }
 
-// Base of z-major neighbour position q = xi + 3*yi + 9*zi (0..26) chosen at run time.
+// Base of z-major neighbour position q = xi + 3*yi + 9*zi (0..26) chosen at
+// run time, mapped from the centre point (nb.xs[1], nb.ys[1], nb.zs[1]).
+// Arithmetic only: a run-time pick among the Nbr fields compiles to an
+// indexed load from a private copy of the struct (scratch memory).
 __device__ __forceinline__ int nbase_q(const StepArgs& a, const Nbr& nb, int q)
 {
     const int xi = q % 3, yi = (q / 3) % 3, zi = q / 9;
-    const int x = xi == 0 ? nb.xs[0] : (xi == 1 ? nb.xs[1] : nb.xs[2]);
-    const int y = yi == 0 ? nb.ys[0] : (yi == 1 ? nb.ys[1] : nb.ys[2]);
-    const int z = zi == 0 ? nb.zs[0] : (zi == 1 ? nb.zs[1] : nb.zs[2]);
+    const int x = bc_map(nb.xs[1] + xi - 1, a.fnx, a.bcx);
+    const int y = bc_map(nb.ys[1] + yi - 1, a.fny, a.bcy);
+    const int z = bc_map(nb.zs[1] + zi - 1, a.fnz, a.bcz);
     return x + y * a.fnx + z * a.fnx * a.fny;
 }
+
+// Rolled runs of z-major positions [from, to) that carry ids only (direct path).
+template <int K>
+__device__ __forceinline__ void scan_point_run(const StepArgs& a, const Nbr& nb, int from, int to,
+                                               Gather<K>& g);
 
 // Stencil row bases D,B,L,C,R,T,U (stencil = -3..3, 3DgrainMPI/APTPhaseField.cu:211-214).
 __device__ __forceinline__ void stencil_bases(const StepArgs& a, const Nbr& nb, int (&b)[7])
@@ -261,6 +269,14 @@ __device__ __forceinline__ void stencil_bases(const StepArgs& a, const Nbr& nb, 
     b[4] = nbase(a, nb, 2, 1, 1);
     b[5] = nbase(a, nb, 1, 2, 1);
     b[6] = nbase(a, nb, 1, 1, 2);
+}
+
+template <int K>
+__device__ __forceinline__ void scan_point_run(const StepArgs& a, const Nbr& nb, int from, int to,
+                                               Gather<K>& g)
+{
+#pragma unroll 1
+    for (int q = from; q < to; ++q) scan_point<K, -1>(a.arg, nbase_q(a, nb, q), a.L, g);
 }
 
 // Pass A over the whole neighbourhood in the reference's scan order.
@@ -280,23 +296,19 @@ __device__ __forceinline__ void scan_all(const StepArgs& a, const Nbr& nb, Gathe
         // q = xi + 3*yi + 9*zi.  The stencil rows keep phi: D=4 B=10 L=12 C=13
         // R=14 T=16 U=22; the other 20 positions only contribute ids and run as
         // rolled loops (same order) to keep the code size down.
-        auto run = [&](int from, int to) {
-#pragma unroll 1
-            for (int q = from; q < to; ++q) scan_point<K, -1>(a.arg, nbase_q(a, nb, q), a.L, g);
-        };
-        run(0, 4);
+        scan_point_run<K>(a, nb, 0, 4, g);
         scan_point<K, 0>(a.arg, nbase_q(a, nb, 4), a.L, g);
-        run(5, 10);
+        scan_point_run<K>(a, nb, 5, 10, g);
         scan_point<K, 1>(a.arg, nbase_q(a, nb, 10), a.L, g);
-        run(11, 12);
+        scan_point_run<K>(a, nb, 11, 12, g);
         scan_point<K, 2>(a.arg, nbase_q(a, nb, 12), a.L, g);
         scan_point<K, 3>(a.arg, nbase_q(a, nb, 13), a.L, g);
         scan_point<K, 4>(a.arg, nbase_q(a, nb, 14), a.L, g);
-        run(15, 16);
+        scan_point_run<K>(a, nb, 15, 16, g);
         scan_point<K, 5>(a.arg, nbase_q(a, nb, 16), a.L, g);
-        run(17, 22);
+        scan_point_run<K>(a, nb, 17, 22, g);
         scan_point<K, 6>(a.arg, nbase_q(a, nb, 22), a.L, g);
-        run(23, 27);
+        scan_point_run<K>(a, nb, 23, 27, g);
     }
 }
 
@@ -611,6 +623,16 @@ __device__ __forceinline__ void scan_point_kept(const int* __restrict__ arg, int
     __builtin_amdgcn_sched_barrier(0);
 }
 
+// Rolled runs of z-major positions [from, to) through pass A restricted to
+// the kept ids (ids only).
+template <int K>
+__device__ __forceinline__ void kept_run(const StepArgs& a, const Nbr& nb, int from, int to,
+                                         Gather<K>& g, const TopK<K>& c)
+{
+#pragma unroll 1
+    for (int q = from; q < to; ++q) scan_point_kept<K, -1>(a.arg, nbase_q(a, nb, q), a.L, g, c);
+}
+
 template <int K, bool G27>
 __device__ __forceinline__ void gather_overflow(const StepArgs& a, const Nbr& nb, Gather<K>& g)
 {
@@ -641,24 +663,19 @@ __device__ __forceinline__ void gather_overflow(const StepArgs& a, const Nbr& nb
     } else {
 #pragma unroll 1
         for (int q = 0; q < 27; ++q) topk_point<K>(a, nbase_q(a, nb, q), q, c);
-        auto run = [&](int from, int to) {
-#pragma unroll 1
-            for (int q = from; q < to; ++q)
-                scan_point_kept<K, -1>(a.arg, nbase_q(a, nb, q), a.L, g, c);
-        };
-        run(0, 4);
+        kept_run<K>(a, nb, 0, 4, g, c);
         scan_point_kept<K, 0>(a.arg, nbase_q(a, nb, 4), a.L, g, c);
-        run(5, 10);
+        kept_run<K>(a, nb, 5, 10, g, c);
         scan_point_kept<K, 1>(a.arg, nbase_q(a, nb, 10), a.L, g, c);
-        run(11, 12);
+        kept_run<K>(a, nb, 11, 12, g, c);
         scan_point_kept<K, 2>(a.arg, nbase_q(a, nb, 12), a.L, g, c);
         scan_point_kept<K, 3>(a.arg, nbase_q(a, nb, 13), a.L, g, c);
         scan_point_kept<K, 4>(a.arg, nbase_q(a, nb, 14), a.L, g, c);
-        run(15, 16);
+        kept_run<K>(a, nb, 15, 16, g, c);
         scan_point_kept<K, 5>(a.arg, nbase_q(a, nb, 16), a.L, g, c);
-        run(17, 22);
+        kept_run<K>(a, nb, 17, 22, g, c);
         scan_point_kept<K, 6>(a.arg, nbase_q(a, nb, 22), a.L, g, c);
-        run(23, 27);
+        kept_run<K>(a, nb, 23, 27, g, c);
     }
 }
 
@@ -857,6 +874,14 @@ __device__ __forceinline__ void scan_lds(const int* __restrict__ plane, int pt, 
     __builtin_amdgcn_sched_barrier(0);
 }
 
+// Rolled runs of z-major positions [from, to) of one plane (ids only).
+template <int K>
+__device__ __forceinline__ void scan_lds_run(const int* pl, int pt, int from, int to, Gather<K>& g)
+{
+#pragma unroll 1
+    for (int q = from; q < to; ++q) scan_lds<K, -1>(pl, pt + ((q / 3) % 3 - 1) * kHX + (q % 3 - 1), g);
+}
+
 template <int K, bool G27>
 __device__ __forceinline__ void scan_all_lds(const int* pD, const int* pC, const int* pU, int pt,
                                              Gather<K>& g)
@@ -887,28 +912,24 @@ __device__ __forceinline__ void scan_all_lds(const int* pD, const int* pC, const
         probe_ids<K, 6>(b0, g);
         __builtin_amdgcn_sched_barrier(0);
     } else {
-        // z-major 27-point order as in scan_all; non-stencil positions rolled
-        auto run = [&](int from, int to) {
-#pragma unroll 1
-            for (int q = from; q < to; ++q) {
-                const int zi = q / 9;
-                const int* pl = zi == 0 ? pD : (zi == 1 ? pC : pU);
-                scan_lds<K, -1>(pl, pt + ((q / 3) % 3 - 1) * kHX + (q % 3 - 1), g);
-            }
-        };
-        run(0, 4);
+        // z-major 27-point order as in scan_all; non-stencil positions rolled,
+        // each run within one plane (a run-time pick among the plane pointers
+        // compiles to an indexed load from a private copy of them)
+        scan_lds_run<K>(pD, pt, 0, 4, g);
         scan_lds<K, 0>(pD, pt, g);
-        run(5, 10);
+        scan_lds_run<K>(pD, pt, 5, 9, g);
+        scan_lds_run<K>(pC, pt, 9, 10, g);
         scan_lds<K, 1>(pC, pt - kHX, g);
-        run(11, 12);
+        scan_lds_run<K>(pC, pt, 11, 12, g);
         scan_lds<K, 2>(pC, pt - 1, g);
         scan_lds<K, 3>(pC, pt, g);
         scan_lds<K, 4>(pC, pt + 1, g);
-        run(15, 16);
+        scan_lds_run<K>(pC, pt, 15, 16, g);
         scan_lds<K, 5>(pC, pt + kHX, g);
-        run(17, 22);
+        scan_lds_run<K>(pC, pt, 17, 18, g);
+        scan_lds_run<K>(pU, pt, 18, 22, g);
         scan_lds<K, 6>(pU, pt, g);
-        run(23, 27);
+        scan_lds_run<K>(pU, pt, 23, 27, g);
     }
 }
 
