@@ -60,12 +60,16 @@ struct StepArgs {
     // Work of one step: nzchunk * edge_items edge items (256 points of the
     // leftover columns each), then the tile chunks listed by the plan kernel:
     // wl_item = z chunk * items_chunk + full tile, with the chunk's plane masks.
-    // wl_count[0] listed tile chunks, wl_count[1] the next work index to take
-    // (words 2 and 3 of this step's counter set).
+    // Chunks with more than half their planes to process are listed from the
+    // front (wl_count[0] of them), the others from the back of the list
+    // (wl_light[0] of them, down from nzchunk * items_chunk - 1), so the long
+    // items are taken first and the short ones fill the end.  wl_count[1] is
+    // the next work index to take (words 2 and 3 of this step's counter set).
     int* wl_item;
     unsigned long long* wl_need;
     unsigned long long* wl_act;
     unsigned* wl_count;
+    unsigned* wl_light;    // chunks listed from the end of the list (word 1 of the counter set)
     int items_chunk;       // full tiles per z chunk
     int edge_cols;         // interior columns outside the full tiles
     int edge_items;        // edge items per z chunk: ceil(kchunk * edge_cols / 256)
@@ -126,8 +130,8 @@ constexpr int kMaxBatch = 1024;
 
 // A batched step over n simulations sharing K and the gather variant.  The
 // work list holds tile chunks of every simulation; cnt holds two 4-word
-// counter sets ([0] listed items, [1] next work index) used by alternate
-// steps, selected by cnt[8].
+// counter sets ([0] chunks listed from the front, [1] next work index, [2]
+// chunks listed from the back) used by alternate steps, selected by cnt[8].
 struct BatchArgs {
     BatchSim* sims;
     int n;
@@ -138,6 +142,7 @@ struct BatchArgs {
     unsigned long long* wl_need;
     unsigned long long* wl_act;
     unsigned* cnt;
+    int wl_cap;            // list capacity: light chunks are listed down from wl_cap - 1
     int step_blocks;
 };
 

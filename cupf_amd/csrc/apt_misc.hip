@@ -12,28 +12,45 @@ namespace {
 // 16 for a batch, whose plan has many more waves.
 constexpr int kBatchPlanBlock = 1024;
 
-// List position of this lane's chunk if `put`; every thread of the workgroup
-// (kPlanWaves waves) calls it.  Waves take consecutive runs of positions in
-// wave order.
+// List position of this lane's chunk if `put`: a heavy chunk (more than half
+// of its planes to process) from the front of the list, a light one from the
+// back (cap - 1 downward).  Every thread of the workgroup (kPlanWaves waves)
+// calls it; waves take consecutive runs of positions in wave order.
 template <int kPlanWaves>
-__device__ __forceinline__ unsigned plan_slot(bool put, unsigned* counter, unsigned* s_cnt)
+__device__ __forceinline__ unsigned plan_slot(bool put, bool heavy, unsigned* front, unsigned* back,
+                                              unsigned cap, unsigned* s_cnt)
 {
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const unsigned long long m = __ballot(put);
-    if (lane == 0) s_cnt[w] = (unsigned)__popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned tot = 0u;
-        for (int u = 0; u < kPlanWaves; ++u) {
-            const unsigned c = s_cnt[u];
-            s_cnt[u] = tot;
-            tot += c;
-        }
-        s_cnt[kPlanWaves] = tot ? atomicAdd(counter, tot) : 0u;
+    const unsigned long long mh = __ballot(put && heavy);
+    const unsigned long long ml = __ballot(put && !heavy);
+    if (lane == 0) {
+        s_cnt[w] = (unsigned)__popcll(mh);
+        s_cnt[kPlanWaves + 1 + w] = (unsigned)__popcll(ml);
     }
     __syncthreads();
-    return s_cnt[kPlanWaves] + s_cnt[w] + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    if (threadIdx.x == 0) {
+        unsigned th = 0u, tl = 0u;
+        for (int u = 0; u < kPlanWaves; ++u) {
+            const unsigned ch = s_cnt[u], cl = s_cnt[kPlanWaves + 1 + u];
+            s_cnt[u] = th;
+            s_cnt[kPlanWaves + 1 + u] = tl;
+            th += ch;
+            tl += cl;
+        }
+        s_cnt[kPlanWaves] = th ? atomicAdd(front, th) : 0u;
+        s_cnt[2 * kPlanWaves + 1] = tl ? atomicAdd(back, tl) : 0u;
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (heavy) return s_cnt[kPlanWaves] + s_cnt[w] + (unsigned)__popcll(mh & below);
+    return cap - 1u - (s_cnt[2 * kPlanWaves + 1] + s_cnt[kPlanWaves + 1 + w] +
+                       (unsigned)__popcll(ml & below));
+}
+
+__device__ __forceinline__ bool plan_heavy(uint64_t need, int kchunk)
+{
+    return 2 * __popcll(need) > kchunk;
 }
 
 // Work list of one step (runs before apt_step_tiled): waves over (z-chunk
@@ -44,7 +61,7 @@ __device__ __forceinline__ unsigned plan_slot(bool put, unsigned* counter, unsig
 __global__ __launch_bounds__(kBlock) void apt_plan(StepArgs a)
 {
     constexpr int kPlanWaves = kBlock / 64;
-    __shared__ unsigned s_cnt[kPlanWaves + 1];
+    __shared__ unsigned s_cnt[2 * kPlanWaves + 2];
     const int g = blockIdx.x * kBlock + threadIdx.x;
     // the counter set of the next step (last used by the previous one, whose
     // fix-up has finished) starts from zero
@@ -56,7 +73,8 @@ __global__ __launch_bounds__(kBlock) void apt_plan(StepArgs a)
     uint64_t need = 0ull, act = 0ull;
     if (gw < plan_waves(a.kchunk, a.nzchunk, a.items_chunk))
         plan_wave(a, gw, lane, put, item, need, act);
-    const unsigned q = plan_slot<kPlanWaves>(put, a.wl_count, s_cnt);
+    const unsigned q = plan_slot<kPlanWaves>(put, plan_heavy(need, a.kchunk), a.wl_count, a.wl_light,
+                                             (unsigned)(a.nzchunk * a.items_chunk), s_cnt);
     if (put) {
         a.wl_item[q] = item;
         a.wl_need[q] = need;
@@ -69,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void apt_plan(StepArgs a)
 __global__ __launch_bounds__(kBatchPlanBlock) void batch_plan(BatchArgs b)
 {
     constexpr int kPlanWaves = kBatchPlanBlock / 64;
-    __shared__ unsigned s_cnt[kPlanWaves + 1];
+    __shared__ unsigned s_cnt[2 * kPlanWaves + 2];
     // wave-uniform index and member records read through the constant address
     // space (batch_advance rewrites them only after this kernel): the search
     // and the step-argument fields are scalar loads
@@ -78,7 +96,7 @@ __global__ __launch_bounds__(kBatchPlanBlock) void batch_plan(BatchArgs b)
     const CSim* sims = (const CSim*)b.sims;
     unsigned* cnt = b.cnt + 4 * __atomic_load_n(b.cnt + 8, __ATOMIC_RELAXED);
     const int lane = threadIdx.x & 63;
-    bool put = false;
+    bool put = false, heavy = false;
     int item = 0, sim = 0;
     uint64_t need = 0ull, act = 0ull;
     if (gw < b.plan_total) {
@@ -91,8 +109,9 @@ __global__ __launch_bounds__(kBatchPlanBlock) void batch_plan(BatchArgs b)
         sim = lo;
         const StepArgs& a = *(const StepArgs*)&sims[sim].a;
         plan_wave(a, gw - sims[sim].plan_base, lane, put, item, need, act);
+        heavy = plan_heavy(need, a.kchunk);
     }
-    const unsigned q = plan_slot<kPlanWaves>(put, cnt, s_cnt);
+    const unsigned q = plan_slot<kPlanWaves>(put, heavy, cnt, cnt + 2, (unsigned)b.wl_cap, s_cnt);
     if (put) {
         b.wl_sim[q] = sim;
         b.wl_item[q] = item;
@@ -112,6 +131,7 @@ __global__ __launch_bounds__(64) void batch_advance(BatchArgs b)
         const unsigned p = b.cnt[8];
         b.cnt[4 * (1 - p)] = 0u;
         b.cnt[4 * (1 - p) + 1] = 0u;
+        b.cnt[4 * (1 - p) + 2] = 0u;
         b.cnt[8] = 1u - p;
     }
     if (s >= b.n) return;
@@ -139,6 +159,7 @@ __global__ __launch_bounds__(64) void batch_advance(BatchArgs b)
     a.fix_count = next;
     a.cnt_next = m.cnt + 4 * (1 - m.parity);
     a.wl_count = next + 2;
+    a.wl_light = next + 1;
     // t_cur_step = (2*kt+1)*dt*tau0 in float (3DgrainMPI/APTPhaseField.cu:863,919)
     a.t = (float)m.step * a.dt * m.tau0;
 }

@@ -128,6 +128,7 @@ StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
         a.fix_count = c->fix_count + 4 * set;
         a.cnt_next = c->fix_count + 4 * (1 - set);
         a.wl_count = a.fix_count + 2;
+        a.wl_light = a.fix_count + 1;
     }
     a.sink_i = c->sink;
     a.sink_f = reinterpret_cast<float*>(c->sink + 64);
@@ -370,6 +371,7 @@ int batch_upload(pf_batch* b, const std::vector<int>& first_step, int use_iuc, B
     ba.wl_item = b->wl_item;
     ba.wl_need = b->wl_need;
     ba.wl_act = b->wl_act;
+    ba.wl_cap = (int)b->wl_cap;
     ba.cnt = b->cnt;
     ba.step_blocks = b->step_blocks;
     return PF_OK;

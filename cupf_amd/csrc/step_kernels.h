@@ -23,7 +23,9 @@ __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
     const float a_s = 1.0f + 3.0f * a.kin_delta;
     const float epsv = -4.0f * a.kin_delta / a_s;
     const unsigned ne = (unsigned)(a.nzchunk * a.edge_items);
-    const unsigned n = ne + __atomic_load_n(a.wl_count, __ATOMIC_RELAXED);
+    const unsigned nh = __atomic_load_n(a.wl_count, __ATOMIC_RELAXED);
+    const unsigned n = ne + nh + __atomic_load_n(a.wl_light, __ATOMIC_RELAXED);
+    const unsigned cap = (unsigned)(a.nzchunk * a.items_chunk);
     for (;;) {
         // the next item is taken only when this workgroup is free, which keeps
         // the tail of the list balanced
@@ -40,7 +42,8 @@ __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
             edge_point<K, G27>(a, e * kBlock + threadIdx.x, k0, k1, a_s, epsv);
         } else {
-            const unsigned w = q - ne;
+            // heavy chunks from the front of the list, then light ones from the back
+            const unsigned w = q - ne < nh ? q - ne : cap - 1u - (q - ne - nh);
             const int g = a.wl_item[w];
             const int zc = g / a.items_chunk;
             const int t = g - zc * a.items_chunk;
@@ -118,14 +121,17 @@ __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
     const CSim* sims = (const CSim*)b.sims;
     const unsigned* cnt = b.cnt + 4 * __atomic_load_n(b.cnt + 8, __ATOMIC_RELAXED);
     const unsigned ne = (unsigned)b.edge_total;
-    const unsigned n = ne + __atomic_load_n(cnt, __ATOMIC_RELAXED);
+    const unsigned nh = __atomic_load_n(cnt, __ATOMIC_RELAXED);
+    const unsigned n = ne + nh + __atomic_load_n(cnt + 2, __ATOMIC_RELAXED);
+    const unsigned cap = (unsigned)b.wl_cap;
     for (;;) {
         if (threadIdx.x == 0) s_q = atomicAdd((unsigned*)cnt + 1, 1u);
         __syncthreads();
         const unsigned q = __builtin_amdgcn_readfirstlane(s_q);
         if (q >= n) break;
         const bool edge = q < ne;
-        const unsigned w = q - ne;
+        // heavy chunks from the front of the list, then light ones from the back
+        const unsigned w = q - ne < nh ? q - ne : cap - 1u - (q - ne - nh);
         const int sim =
             __builtin_amdgcn_readfirstlane(edge ? batch_find(b.sims, b.n, (int)q, true) : b.wl_sim[w]);
         const StepArgs& a = *(const StepArgs*)&sims[sim].a;
