@@ -1060,9 +1060,10 @@ __device__ __forceinline__ void plan_wave(const StepArgs& a, int gw, int lane, b
             const int rl = yl * a.ntx, rc = ty * a.ntx, rr = yr * a.ntx;
             const unsigned* w = a.act_src + bc_map(kk, a.fnz, a.bcz) * a.ntiles;
             const unsigned e = a.ep_src;
-            on = w[xl + rl] >= e || w[tx + rl] >= e || w[xr + rl] >= e || w[xl + rc] >= e ||
-                 w[tx + rc] >= e || w[xr + rc] >= e || w[xl + rr] >= e || w[tx + rr] >= e ||
-                 w[xr + rr] >= e;
+            // bitwise ors: the nine reads go out together
+            on = (w[xl + rl] >= e) | (w[tx + rl] >= e) | (w[xr + rl] >= e) | (w[xl + rc] >= e) |
+                 (w[tx + rc] >= e) | (w[xr + rc] >= e) | (w[xl + rr] >= e) | (w[tx + rr] >= e) |
+                 (w[xr + rr] >= e);
             dst = kk >= kb && kk < klast && a.act_dst[kk * a.ntiles + tx + rc] >= a.ep_dst;
         }
     }
@@ -1166,14 +1167,17 @@ __device__ __forceinline__ void edge_point(const StepArgs& a, int p, int k0, int
         const int rl = act_tile_y(bc_map(j - 1, a.fny, a.bcy)) * a.ntx, rc = act_tile_y(j) * a.ntx;
         const int rr = act_tile_y(bc_map(j + 1, a.fny, a.bcy)) * a.ntx;
         const unsigned e = a.ep_src;
-        need = a.act_dst[k * a.ntiles + tile] >= a.ep_dst;
+        // bitwise ors: all 28 reads go out together (a short-circuit chain
+        // would issue them one memory latency apart)
+        bool any = a.act_dst[k * a.ntiles + tile] >= a.ep_dst;
 #pragma unroll
         for (int dz = -1; dz <= 1; ++dz) {
             const unsigned* w = a.act_src + bc_map(k + dz, a.fnz, a.bcz) * a.ntiles;
-            need = need || w[xl + rl] >= e || w[xc + rl] >= e || w[xr + rl] >= e ||
-                   w[xl + rc] >= e || w[xc + rc] >= e || w[xr + rc] >= e ||
-                   w[xl + rr] >= e || w[xc + rr] >= e || w[xr + rr] >= e;
+            any = any | (w[xl + rl] >= e) | (w[xc + rl] >= e) | (w[xr + rl] >= e) |
+                  (w[xl + rc] >= e) | (w[xc + rc] >= e) | (w[xr + rc] >= e) |
+                  (w[xl + rr] >= e) | (w[xc + rr] >= e) | (w[xr + rr] >= e);
         }
+        need = any;
     }
     int res = 0;
     if (need) {

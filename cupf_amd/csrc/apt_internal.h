@@ -57,8 +57,8 @@ struct StepArgs {
     unsigned ep_new;       // epoch the destination buffer gets from this step
     int use_act;           // 0: process every plane (the map is still maintained)
     int ntx, nty, ntiles;  // activity tiles (64x4, partial ones included)
-    // Work of one step: nzchunk * edge_items edge items (256 points of the
-    // leftover columns each), then the tile chunks listed by the plan kernel:
+    // Work of one step: nzchunk * edge_items edge items (kEdgePoints points
+    // of the leftover columns each), then the tile chunks listed by the plan kernel:
     // wl_item = z chunk * items_chunk + full tile, with the chunk's plane masks.
     // Chunks with more than half their planes to process are listed from the
     // front (wl_count[0] of them), the others from the back of the list
@@ -72,7 +72,7 @@ struct StepArgs {
     unsigned* wl_light;    // chunks listed from the end of the list (word 1 of the counter set)
     int items_chunk;       // full tiles per z chunk
     int edge_cols;         // interior columns outside the full tiles
-    int edge_items;        // edge items per z chunk: ceil(kchunk * edge_cols / 256)
+    int edge_items;        // edge items per z chunk: ceil(kchunk * edge_cols / kEdgePoints)
     int step_blocks;       // persistent workgroups of the step kernel
     int fnx, fny, fnz;
     int L;                 // slot stride fnx*fny*fnz
@@ -124,6 +124,12 @@ __host__ __device__ inline int plan_waves(int kchunk, int nzchunk, int items_chu
     const int g = plan_group(kchunk), tw = plan_tiles(kchunk);
     return ((nzchunk + g - 1) / g) * ((items_chunk + tw - 1) / tw);
 }
+
+// Points of one edge item: kEdgePerLane per lane of a 256-lane workgroup
+// (4 per lane was measured 8 % slower on the 256x256x512 step: edge items are
+// taken first, and fewer, longer ones hold workgroups the tile items need).
+constexpr int kEdgePerLane = 1;
+constexpr int kEdgePoints = 256 * kEdgePerLane;
 
 // Most simulations one batch may hold.
 constexpr int kMaxBatch = 1024;

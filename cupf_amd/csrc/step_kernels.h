@@ -40,7 +40,9 @@ __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
             const int e = (int)q - zc * a.edge_items;
             const int k0 = 1 + zc * a.kchunk;
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-            edge_point<K, G27>(a, e * kBlock + threadIdx.x, k0, k1, a_s, epsv);
+#pragma unroll 1
+            for (int r = 0; r < kEdgePerLane; ++r)
+                edge_point<K, G27>(a, e * kEdgePoints + r * kBlock + threadIdx.x, k0, k1, a_s, epsv);
         } else {
             // heavy chunks from the front of the list, then light ones from the back
             const unsigned w = q - ne < nh ? q - ne : cap - 1u - (q - ne - nh);
@@ -143,7 +145,9 @@ __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
             const int e = e0 - zc * a.edge_items;
             const int k0 = 1 + zc * a.kchunk;
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-            edge_point<K, G27>(a, e * kBlock + threadIdx.x, k0, k1, a_s, epsv);
+#pragma unroll 1
+            for (int r = 0; r < kEdgePerLane; ++r)
+                edge_point<K, G27>(a, e * kEdgePoints + r * kBlock + threadIdx.x, k0, k1, a_s, epsv);
         } else {
             const int g = b.wl_item[w];
             const int zc = g / a.items_chunk;
