@@ -104,6 +104,8 @@ def load() -> ctypes.CDLL:
         "pf_step_timed": ([vp, c_int, c_int, c_int, fp], c_int),
         "pf_step_profiled": ([vp, c_int, c_int, c_int, fp, fp, fp], c_int),
         "pf_collect": ([vp, c_int, c_int, fp, ip], c_int),
+        "pf_qoi_volume": ([vp, c_int, ip, ip, llp], c_int),
+        "pf_qoi_junctions3d": ([vp, ip, c_int, ip], c_int),
         "pf_md_enable": ([vp, c_int], c_int),
         "pf_md_tip": ([vp, ip], c_int),
         "pf_md_shift": ([vp, c_int], c_int),
@@ -252,6 +254,33 @@ class APTContext:
         alpha = np.empty(self.L, dtype=np.int32)
         _check(load().pf_collect(self._h, which, flags, _fptr(phi), _iptr(alpha)))
         return phi, alpha
+
+    def collect_on_device(self, which=0, flags=0):
+        """pf_collect keeping phi/alpha on the device (for the device QoIs)."""
+        _check(load().pf_collect(self._h, which, flags, None, None))
+
+    def qoi_volume(self, max_id: int):
+        """Device QOI3D::calculateQoIs on the collected alpha: (ids in
+        first-seen scan order, their point counts, points with an id above
+        max_id)."""
+        counts = np.zeros(max_id + 1, dtype=np.int32)
+        first = np.zeros(max_id + 1, dtype=np.int32)
+        above = ctypes.c_longlong()
+        _check(load().pf_qoi_volume(self._h, max_id, _iptr(counts), _iptr(first), ctypes.byref(above)))
+        ids = np.nonzero(counts)[0]
+        ids = ids[np.argsort(first[ids], kind="stable")]
+        return ids.tolist(), counts[ids].tolist(), above.value
+
+    def qoi_junctions3d(self):
+        """Device QOI3D::searchJunctionsOnImage on the collected alpha: rows
+        (i, j, k, largest count, (4 ascending ids)) in scan order."""
+        n = ctypes.c_int()
+        _check(load().pf_qoi_junctions3d(self._h, None, 0, ctypes.byref(n)))
+        cap = n.value
+        rows = np.zeros(8 * max(cap, 1), dtype=np.int32)
+        _check(load().pf_qoi_junctions3d(self._h, _iptr(rows), cap, ctypes.byref(n)))
+        r = rows[:8 * min(cap, n.value)].reshape(-1, 8)
+        return [(int(x[0]), int(x[1]), int(x[2]), int(x[3]), tuple(int(v) for v in x[4:])) for x in r]
 
     # --- moving domain / nucleation / diagnostics --------------------------------
     def md_enable(self, fnz_full: int):

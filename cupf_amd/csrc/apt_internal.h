@@ -210,6 +210,11 @@ int step_slots_k8_g27(int device, bool batch);
 hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,
                        hipEvent_t ev_begin = nullptr, hipEvent_t ev_planned = nullptr,
                        hipEvent_t ev_mid = nullptr, hipEvent_t ev_end = nullptr);
+// Device-side QoIs on a collected alpha field (qoi.hip).
+hipError_t launch_qoi_volume(const int* alpha, int fnx, int fny, int fnz, int max_id, int* counts,
+                             int* first, unsigned long long* above, hipStream_t stream);
+hipError_t launch_qoi_junctions(const int* alpha, int fnx, int fny, int fnz, int* rows, int cap,
+                                unsigned* count, hipStream_t stream);
 hipError_t launch_bc_fill(float* ph, int* arg, int fnx, int fny, int fnz, int K, int bcx,
                           int bcy, int bcz, hipStream_t stream);
 hipError_t launch_collect(const float* ph, const int* arg, float* phi, int* alpha, int L, int K,

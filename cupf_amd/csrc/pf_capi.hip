@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <tuple>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -816,6 +818,71 @@ int pf_collect(pf_ctx* c, int which, int flags, float* phi_out, int* alpha_out)
         HIPCHK(hipMemcpyAsync(alpha_out, c->alpha, sizeof(int) * c->L, hipMemcpyDeviceToHost,
                               c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_qoi_volume(pf_ctx* c, int max_id, int* counts, int* first, long long* n_above)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (max_id < 1 || !counts || !first) return fail(PF_E_INVALID, "max_id >= 1, counts and first required");
+    const pf_params& p = c->p;
+    int *dc = nullptr, *df = nullptr;
+    unsigned long long* da = nullptr;
+    if ((rc = dalloc(&dc, (size_t)max_id + 1)) || (rc = dalloc(&df, (size_t)max_id + 1)) ||
+        (rc = dalloc(&da, 1))) {
+        (void)hipFree(dc);
+        (void)hipFree(df);
+        return rc;
+    }
+    unsigned long long above = 0;
+    hipError_t e = launch_qoi_volume(c->alpha, p.fnx, p.fny, p.fnz, max_id, dc, df, da, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(counts, dc, sizeof(int) * ((size_t)max_id + 1), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(first, df, sizeof(int) * ((size_t)max_id + 1), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&above, da, sizeof(above), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dc);
+    (void)hipFree(df);
+    (void)hipFree(da);
+    if (e != hipSuccess) return fail(PF_E_HIP, std::string("qoi volume: ") + hipGetErrorString(e));
+    if (n_above) *n_above = (long long)above;
+    return PF_OK;
+}
+
+int pf_qoi_junctions3d(pf_ctx* c, int* rows, int cap, int* count)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (cap < 0 || (cap > 0 && !rows) || !count) return fail(PF_E_INVALID, "bad rows/cap/count");
+    const pf_params& p = c->p;
+    int* dr = nullptr;
+    unsigned* dn = nullptr;
+    if ((rc = dalloc(&dr, (size_t)8 * std::max(cap, 1))) || (rc = dalloc(&dn, 1))) {
+        (void)hipFree(dr);
+        return rc;
+    }
+    unsigned n = 0;
+    hipError_t e = launch_qoi_junctions(c->alpha, p.fnx, p.fny, p.fnz, dr, cap, dn, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&n, dn, sizeof(n), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    const int m = (int)std::min<unsigned>(n, (unsigned)cap);
+    if (e == hipSuccess && m > 0)
+        e = hipMemcpy(rows, dr, sizeof(int) * 8 * (size_t)m, hipMemcpyDeviceToHost);
+    (void)hipFree(dr);
+    (void)hipFree(dn);
+    if (e != hipSuccess) return fail(PF_E_HIP, std::string("qoi junctions: ") + hipGetErrorString(e));
+    // rows were appended in any order: the reference's scan order is (k, j, i)
+    std::vector<std::array<int, 8>> v((size_t)m);
+    for (int q = 0; q < m; ++q)
+        for (int u = 0; u < 8; ++u) v[q][u] = rows[8 * (size_t)q + u];
+    std::sort(v.begin(), v.end(), [](const std::array<int, 8>& x, const std::array<int, 8>& y) {
+        return std::make_tuple(x[2], x[1], x[0]) < std::make_tuple(y[2], y[1], y[0]);
+    });
+    for (int q = 0; q < m; ++q)
+        for (int u = 0; u < 8; ++u) rows[8 * (size_t)q + u] = v[q][u];
+    *count = (int)n;
     return PF_OK;
 }
 

@@ -1,0 +1,156 @@
+// qoi.hip -- device-side quantities of interest on the collected grain-id
+// field (alpha, fnx*fny*fnz with its halo, as pf_collect leaves it): per-grain
+// volumes (QOI3D::calculateQoIs, 3DgrainMPI/QOI.cpp:245-278) and the 3-D
+// junction search (QOI3D::searchJunctionsOnImage, 3DgrainMPI/QOI.cpp:280-358).
+// Only the results cross PCIe, not the field.
+#include "apt_device.h"
+
+namespace cupf {
+
+namespace {
+
+// Per-grain point counts and first occurrence over interior points: counts[a]
+// += 1 and first[a] = min scan index for 1 <= a <= max_id, where the scan
+// index runs k-major, then j, then i (the reference's loop order, so sorting
+// ids by first[] reproduces its first-seen order).  Points with a larger id
+// are counted in *above.
+__global__ __launch_bounds__(kBlock) void qoi_volume_kernel(const int* __restrict__ alpha, int fnx,
+                                                            int fny, int fnz, int max_id,
+                                                            int* counts, int* first,
+                                                            unsigned long long* above)
+{
+    const int nx = fnx - 2, ny = fny - 2, nz = fnz - 2;
+    const long long n = (long long)nx * ny * nz;
+    unsigned long long over = 0ull;
+    for (long long s = (long long)blockIdx.x * kBlock + threadIdx.x; s < n;
+         s += (long long)gridDim.x * kBlock) {
+        const int i = 1 + (int)(s % nx);
+        const long long r = s / nx;
+        const int j = 1 + (int)(r % ny);
+        const int k = 1 + (int)(r / ny);
+        const int a = alpha[(long long)k * fnx * fny + (long long)j * fnx + i];
+        if (a <= 0) continue;
+        if (a > max_id) {
+            ++over;
+            continue;
+        }
+        atomicAdd(&counts[a], 1);
+        atomicMin(&first[a], (int)s);
+    }
+    if (over) atomicAdd(above, over);
+}
+
+// 3-D junctions: a solid interior point whose 19-point neighbourhood (offsets
+// with di^2 + dj^2 + dk^2 <= 2, the point itself included) holds no liquid
+// (alpha 0) and exactly 4 distinct ids, none more than 10 times.  Rows of 8
+// ints (i, j, k, largest count, the 4 ids ascending) are appended in any
+// order; the host sorts them into scan order.
+__global__ __launch_bounds__(kBlock) void qoi_junctions_kernel(const int* __restrict__ alpha, int fnx,
+                                                               int fny, int fnz, int* rows, int cap,
+                                                               unsigned* count)
+{
+    const int nx = fnx - 2, ny = fny - 2, nz = fnz - 2;
+    const long long n = (long long)nx * ny * nz;
+    const long long A = (long long)fnx * fny;
+    for (long long s = (long long)blockIdx.x * kBlock + threadIdx.x; s < n;
+         s += (long long)gridDim.x * kBlock) {
+        const int i = 1 + (int)(s % nx);
+        const long long r = s / nx;
+        const int j = 1 + (int)(r % ny);
+        const int k = 1 + (int)(r / ny);
+        const long long C = k * A + (long long)j * fnx + i;
+        if (alpha[C] == 0) continue;
+        int id[4] = {0, 0, 0, 0}, cnt[4] = {0, 0, 0, 0};
+        int nd = 0;
+        bool ok = true;
+#pragma unroll
+        for (int dk = -1; dk <= 1; ++dk)
+#pragma unroll
+            for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+                for (int di = -1; di <= 1; ++di) {
+                    if (di * di + dj * dj + dk * dk > 2) continue;
+                    const int a = alpha[C + dk * A + (long long)dj * fnx + di];
+                    ok = ok && a != 0;
+                    bool hit = false;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const bool h = q < nd && id[q] == a;
+                        cnt[q] += h ? 1 : 0;
+                        hit = hit || h;
+                    }
+                    if (!hit) {
+                        ok = ok && nd < 4;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const bool t = q == nd;
+                            id[q] = t ? a : id[q];
+                            cnt[q] = t ? 1 : cnt[q];
+                        }
+                        nd = nd < 4 ? nd + 1 : nd;
+                    }
+                }
+        if (!ok || nd != 4) continue;
+        int mx = cnt[0];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) mx = cnt[q] > mx ? cnt[q] : mx;
+        if (mx > 10) continue;
+        // 4-element sorting network on the ids
+        auto cs = [](int& x, int& y) {
+            const int lo = x < y ? x : y, hi = x < y ? y : x;
+            x = lo;
+            y = hi;
+        };
+        cs(id[0], id[1]);
+        cs(id[2], id[3]);
+        cs(id[0], id[2]);
+        cs(id[1], id[3]);
+        cs(id[1], id[2]);
+        const unsigned q = atomicAdd(count, 1u);
+        if (q < (unsigned)cap) {
+            int* o = rows + 8 * (size_t)q;
+            o[0] = i;
+            o[1] = j;
+            o[2] = k;
+            o[3] = mx;
+            o[4] = id[0];
+            o[5] = id[1];
+            o[6] = id[2];
+            o[7] = id[3];
+        }
+    }
+}
+
+// first[] starts at INT_MAX (no occurrence).
+__global__ __launch_bounds__(kBlock) void qoi_fill_kernel(int* p, int n, int v)
+{
+    for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) p[q] = v;
+}
+
+}  // namespace
+
+hipError_t launch_qoi_volume(const int* alpha, int fnx, int fny, int fnz, int max_id, int* counts,
+                             int* first, unsigned long long* above, hipStream_t stream)
+{
+    const int nb = max(1, min(2048, cdiv(max_id + 1, kBlock)));
+    hipLaunchKernelGGL(qoi_fill_kernel, dim3(nb), dim3(kBlock), 0, stream, counts, max_id + 1, 0);
+    hipLaunchKernelGGL(qoi_fill_kernel, dim3(nb), dim3(kBlock), 0, stream, first, max_id + 1,
+                       0x7fffffff);
+    hipLaunchKernelGGL(qoi_fill_kernel, dim3(1), dim3(kBlock), 0, stream, (int*)above, 2, 0);
+    const long long n = (long long)(fnx - 2) * (fny - 2) * (fnz - 2);
+    hipLaunchKernelGGL(qoi_volume_kernel, dim3(max(1, min(8192, cdiv(n, kBlock)))), dim3(kBlock), 0,
+                       stream, alpha, fnx, fny, fnz, max_id, counts, first, above);
+    return hipGetLastError();
+}
+
+hipError_t launch_qoi_junctions(const int* alpha, int fnx, int fny, int fnz, int* rows, int cap,
+                                unsigned* count, hipStream_t stream)
+{
+    hipLaunchKernelGGL(qoi_fill_kernel, dim3(1), dim3(kBlock), 0, stream, (int*)count, 1, 0);
+    const long long n = (long long)(fnx - 2) * (fny - 2) * (fnz - 2);
+    hipLaunchKernelGGL(qoi_junctions_kernel, dim3(max(1, min(8192, cdiv(n, kBlock)))), dim3(kBlock),
+                       0, stream, alpha, fnx, fny, fnz, rows, cap, count);
+    return hipGetLastError();
+}
+
+}  // namespace cupf

@@ -149,6 +149,25 @@ int pf_step_profiled(pf_ctx* ctx, int nsteps, int first_step, int flags, float* 
  * NULL to keep the result on the device. */
 int pf_collect(pf_ctx* ctx, int which, int flags, float* phi_out, int* alpha_out);
 
+/* Device-side QoIs on the context's alpha field as the last pf_collect left
+ * it; only the results are copied to the host.
+ *
+ * QOI3D::calculateQoIs (3DgrainMPI/QOI.cpp:245-278): for grain ids
+ * 1..max_id, counts[id] = interior points with that alpha and first[id] =
+ * scan index (k-major, then j, then i, from 0) of the first such point, or
+ * INT_MAX if none; ordering the ids with a count by first[] gives the
+ * reference's first-seen order.  counts and first hold max_id + 1 ints;
+ * *n_above = interior points whose alpha exceeds max_id (nucleated ids). */
+int pf_qoi_volume(pf_ctx* ctx, int max_id, int* counts, int* first, long long* n_above);
+
+/* QOI3D::searchJunctionsOnImage (3DgrainMPI/QOI.cpp:280-358): solid interior
+ * points whose 19-point neighbourhood holds no liquid and exactly 4 distinct
+ * ids, none more than 10 times.  Up to cap rows of 8 ints (i, j, k, largest
+ * count, the 4 ids ascending) in the reference's scan order (k, then j, then
+ * i); *count = junctions found (rows past cap are not written).  The ids are
+ * sorted because the reference's order comes from unordered_map iteration. */
+int pf_qoi_junctions3d(pf_ctx* ctx, int* rows, int cap, int* count);
+
 /* Liquid-start re-initialisation (3DgrainMPI/APTPhaseField.cu:809-818): both
  * buffers set to -1, then slot 0 = (initial phi0, collected alpha) where
  * phi0 > -0.995.  Follows a pf_collect with PF_COLLECT_CLEAR_ALPHA. */

@@ -1,0 +1,64 @@
+"""Device-side QoIs (pf_qoi_volume, pf_qoi_junctions3d) against the numpy
+restatement of 3DgrainMPI/QOI.cpp in oracle/qoi.py, on the same collected
+alpha field: a simulated state, and an uploaded block pattern with many
+4-grain junctions and a liquid region."""
+import numpy as np
+import pytest
+
+from cupf_amd import capi, cases
+from oracle import qoi
+from tests.helpers import gpu_context
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(ctx, p, max_id=None):
+    """Host alpha (for the restatement) and the device QoIs of the same field."""
+    _, alpha = ctx.collect(0, capi.PF_COLLECT_CLEAR_ALPHA)
+    ctx.collect_on_device(0, capi.PF_COLLECT_CLEAR_ALPHA)
+    fnx, fny, fnz = int(p["fnx"]), int(p["fny"]), int(p["fnz"])
+    if max_id is None:
+        max_id = max(1, int(alpha.max()))
+    return alpha, (fnx, fny, fnz), ctx.qoi_volume(max_id), ctx.qoi_junctions3d()
+
+
+def test_device_qoi_on_simulated_state():
+    c = cases.make_line_case("qoi", n=(48, 40, 32), grains=40, K=8, seed=31, G=1.0, R=0.1,
+                             z0_cells=20)
+    ctx = gpu_context(c)
+    ctx.step(40, 1)
+    alpha, dims, (ids, counts, above), rows = _both(ctx, c.params)
+    ref_ids, ref_counts = qoi.volume(alpha, *dims)
+    assert above == 0
+    assert ids == ref_ids and counts == ref_counts
+    assert rows == qoi.junctions3d(alpha, *dims)
+
+
+def test_device_qoi_on_block_pattern():
+    c = cases.make_line_case("qoi-b", n=(40, 36, 30), grains=8, K=8, seed=32)
+    ctx = gpu_context(c)
+    p = c.params
+    fnx, fny, fnz, K = int(p["fnx"]), int(p["fny"]), int(p["fnz"]), 8
+    L = fnx * fny * fnz
+    k, j, i = np.meshgrid(np.arange(fnz), np.arange(fny), np.arange(fnx), indexing="ij")
+    rng = np.random.default_rng(5)
+    table = rng.integers(1, 40, size=(fnz // 4 + 1, fny // 3 + 1, fnx // 3 + 1))
+    ids = table[k // 4, j // 3, i // 3].astype(np.int32)
+    liquid = (k > 2 * fnz // 3) & (i < fnx // 2)
+    ph = np.full((K, L), -1.0, dtype=np.float32)
+    arg = np.full((K, L), -1, dtype=np.int32)
+    ph[0] = np.where(liquid, -1.0, 1.0).ravel()
+    arg[0] = np.where(liquid, -1, ids).ravel()
+    ctx.upload_state(ph, arg, 0)
+    # ids up to 39: count them all, then with a bound that leaves some above
+    alpha, dims, (vid, vcnt, above), rows = _both(ctx, p, 64)
+    ref_ids, ref_counts = qoi.volume(alpha, *dims)
+    assert above == 0 and vid == ref_ids and vcnt == ref_counts
+    ref_rows = qoi.junctions3d(alpha, *dims)
+    assert len(ref_rows) > 0
+    assert rows == ref_rows
+    ids20, cnt20, above20 = ctx.qoi_volume(20)
+    a = alpha.reshape(fnz, fny, fnx)[1:-1, 1:-1, 1:-1]
+    assert above20 == int((a > 20).sum())
+    keep = [(g, n) for g, n in zip(ref_ids, ref_counts) if g <= 20]
+    assert list(zip(ids20, cnt20)) == keep
