@@ -106,6 +106,7 @@ def load() -> ctypes.CDLL:
         "pf_collect": ([vp, c_int, c_int, fp, ip], c_int),
         "pf_qoi_volume": ([vp, c_int, ip, ip, llp], c_int),
         "pf_qoi_junctions3d": ([vp, ip, c_int, ip], c_int),
+        "pf_qoi_line": ([vp, c_int, ip, ip, ip], c_int),
         "pf_md_enable": ([vp, c_int], c_int),
         "pf_md_tip": ([vp, ip], c_int),
         "pf_md_shift": ([vp, c_int], c_int),
@@ -270,6 +271,16 @@ class APTContext:
         ids = np.nonzero(counts)[0]
         ids = ids[np.argsort(first[ids], kind="stable")]
         return ids.tolist(), counts[ids].tolist(), above.value
+
+    def qoi_line(self, cur_tip: int):
+        """Device line QoIs on the collected alpha: (new tip plane, total and
+        extra point counts per grain id 1..num_theta)."""
+        ng = int(self.params["num_theta"])
+        tip = ctypes.c_int()
+        total = np.zeros(ng, dtype=np.int32)
+        extra = np.zeros(ng, dtype=np.int32)
+        _check(load().pf_qoi_line(self._h, cur_tip, ctypes.byref(tip), _iptr(total), _iptr(extra)))
+        return tip.value, total, extra
 
     def qoi_junctions3d(self):
         """Device QOI3D::searchJunctionsOnImage on the collected alpha: rows

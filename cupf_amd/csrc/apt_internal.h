@@ -215,6 +215,10 @@ hipError_t launch_qoi_volume(const int* alpha, int fnx, int fny, int fnz, int ma
                              int* first, unsigned long long* above, hipStream_t stream);
 hipError_t launch_qoi_junctions(const int* alpha, int fnx, int fny, int fnz, int* rows, int cap,
                                 unsigned* count, hipStream_t stream);
+hipError_t launch_qoi_liquid_planes(const int* alpha, int fnx, int fny, int fnz, unsigned* liq,
+                                   hipStream_t stream);
+hipError_t launch_qoi_area(const int* alpha, int fnx, int fny, int fnz, int ng, int tip, int* total,
+                           int* extra, hipStream_t stream);
 hipError_t launch_bc_fill(float* ph, int* arg, int fnx, int fny, int fnz, int K, int bcx,
                           int bcy, int bcz, hipStream_t stream);
 hipError_t launch_collect(const float* ph, const int* arg, float* phi, int* alpha, int L, int K,

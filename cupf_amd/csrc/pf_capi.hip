@@ -886,6 +886,50 @@ int pf_qoi_junctions3d(pf_ctx* c, int* rows, int cap, int* count)
     return PF_OK;
 }
 
+int pf_qoi_line(pf_ctx* c, int cur_tip, int* new_tip, int* total, int* extra)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    const pf_params& p = c->p;
+    if (!new_tip || !total || !extra) return fail(PF_E_INVALID, "new_tip, total and extra required");
+    if (cur_tip < 0 || cur_tip > p.fnz - 2) return fail(PF_E_INVALID, "cur_tip out of range");
+    const int ng = p.num_theta;
+    unsigned* dl = nullptr;
+    int *dt = nullptr, *de = nullptr;
+    if ((rc = dalloc(&dl, (size_t)p.fnz)) || (rc = dalloc(&dt, (size_t)ng)) ||
+        (rc = dalloc(&de, (size_t)ng))) {
+        (void)hipFree(dl);
+        (void)hipFree(dt);
+        return rc;
+    }
+    std::vector<unsigned> liq((size_t)p.fnz, 0u);
+    hipError_t e = launch_qoi_liquid_planes(c->alpha, p.fnx, p.fny, p.fnz, dl, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(liq.data(), dl, sizeof(unsigned) * p.fnz, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    int tip = cur_tip;
+    if (e == hipSuccess) {
+        // 3DgrainMPI/QOI.cpp:47-67: step up to the first plane holding liquid
+        for (;;) {
+            ++tip;
+            if (tip >= p.fnz - 1 || liq[tip]) break;
+        }
+        tip -= 1;
+        e = launch_qoi_area(c->alpha, p.fnx, p.fny, p.fnz, ng, tip, dt, de, c->stream);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(total, dt, sizeof(int) * ng, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(extra, de, sizeof(int) * ng, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dl);
+    (void)hipFree(dt);
+    (void)hipFree(de);
+    if (e != hipSuccess) return fail(PF_E_HIP, std::string("qoi line: ") + hipGetErrorString(e));
+    *new_tip = tip;
+    return PF_OK;
+}
+
 int pf_md_enable(pf_ctx* c, int fnz_full)
 {
     int rc = ctx_check(c);

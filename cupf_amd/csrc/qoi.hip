@@ -1,8 +1,9 @@
 // qoi.hip -- device-side quantities of interest on the collected grain-id
 // field (alpha, fnx*fny*fnz with its halo, as pf_collect leaves it): per-grain
-// volumes (QOI3D::calculateQoIs, 3DgrainMPI/QOI.cpp:245-278) and the 3-D
-// junction search (QOI3D::searchJunctionsOnImage, 3DgrainMPI/QOI.cpp:280-358).
-// Only the results cross PCIe, not the field.
+// volumes (QOI3D::calculateQoIs, 3DgrainMPI/QOI.cpp:245-278), the 3-D
+// junction search (QOI3D::searchJunctionsOnImage, 3DgrainMPI/QOI.cpp:280-358)
+// and the line QoIs' tip scan and grain areas (QOI::calculateLineQoIs,
+// 3DgrainMPI/QOI.cpp:38-95).  Only the results cross PCIe, not the field.
 #include "apt_device.h"
 
 namespace cupf {
@@ -121,6 +122,45 @@ __global__ __launch_bounds__(kBlock) void qoi_junctions_kernel(const int* __rest
     }
 }
 
+// Planes whose interior holds a liquid point (alpha 0): liq[k] = 1.  Every
+// writer stores the same value, so plain stores suffice.
+__global__ __launch_bounds__(kBlock) void qoi_liquid_planes_kernel(const int* __restrict__ alpha,
+                                                                   int fnx, int fny, int fnz,
+                                                                   unsigned* liq)
+{
+    const int nx = fnx - 2, ny = fny - 2, nz = fnz - 2;
+    const long long n = (long long)nx * ny * nz;
+    for (long long s = (long long)blockIdx.x * kBlock + threadIdx.x; s < n;
+         s += (long long)gridDim.x * kBlock) {
+        const int i = 1 + (int)(s % nx);
+        const long long r = s / nx;
+        const int j = 1 + (int)(r % ny);
+        const int k = 1 + (int)(r / ny);
+        if (alpha[(long long)k * fnx * fny + (long long)j * fnx + i] == 0) liq[k] = 1u;
+    }
+}
+
+// Line QoIs (QOI::calculateLineQoIs, 3DgrainMPI/QOI.cpp:71-95): per grain id
+// 1..ng, interior points (total) and those above the tip plane (extra).
+__global__ __launch_bounds__(kBlock) void qoi_area_kernel(const int* __restrict__ alpha, int fnx,
+                                                          int fny, int fnz, int ng, int tip,
+                                                          int* total, int* extra)
+{
+    const int nx = fnx - 2, ny = fny - 2, nz = fnz - 2;
+    const long long n = (long long)nx * ny * nz;
+    for (long long s = (long long)blockIdx.x * kBlock + threadIdx.x; s < n;
+         s += (long long)gridDim.x * kBlock) {
+        const int i = 1 + (int)(s % nx);
+        const long long r = s / nx;
+        const int j = 1 + (int)(r % ny);
+        const int k = 1 + (int)(r / ny);
+        const int a = alpha[(long long)k * fnx * fny + (long long)j * fnx + i];
+        if (a <= 0 || a > ng) continue;
+        atomicAdd(&total[a - 1], 1);
+        if (k > tip) atomicAdd(&extra[a - 1], 1);
+    }
+}
+
 // first[] starts at INT_MAX (no occurrence).
 __global__ __launch_bounds__(kBlock) void qoi_fill_kernel(int* p, int n, int v)
 {
@@ -140,6 +180,29 @@ hipError_t launch_qoi_volume(const int* alpha, int fnx, int fny, int fnz, int ma
     const long long n = (long long)(fnx - 2) * (fny - 2) * (fnz - 2);
     hipLaunchKernelGGL(qoi_volume_kernel, dim3(max(1, min(8192, cdiv(n, kBlock)))), dim3(kBlock), 0,
                        stream, alpha, fnx, fny, fnz, max_id, counts, first, above);
+    return hipGetLastError();
+}
+
+hipError_t launch_qoi_liquid_planes(const int* alpha, int fnx, int fny, int fnz, unsigned* liq,
+                                   hipStream_t stream)
+{
+    hipLaunchKernelGGL(qoi_fill_kernel, dim3(max(1, cdiv(fnz, kBlock))), dim3(kBlock), 0, stream,
+                       (int*)liq, fnz, 0);
+    const long long n = (long long)(fnx - 2) * (fny - 2) * (fnz - 2);
+    hipLaunchKernelGGL(qoi_liquid_planes_kernel, dim3(max(1, min(8192, cdiv(n, kBlock)))),
+                       dim3(kBlock), 0, stream, alpha, fnx, fny, fnz, liq);
+    return hipGetLastError();
+}
+
+hipError_t launch_qoi_area(const int* alpha, int fnx, int fny, int fnz, int ng, int tip, int* total,
+                           int* extra, hipStream_t stream)
+{
+    const int nb = max(1, min(2048, cdiv(ng, kBlock)));
+    hipLaunchKernelGGL(qoi_fill_kernel, dim3(nb), dim3(kBlock), 0, stream, total, ng, 0);
+    hipLaunchKernelGGL(qoi_fill_kernel, dim3(nb), dim3(kBlock), 0, stream, extra, ng, 0);
+    const long long n = (long long)(fnx - 2) * (fny - 2) * (fnz - 2);
+    hipLaunchKernelGGL(qoi_area_kernel, dim3(max(1, min(8192, cdiv(n, kBlock)))), dim3(kBlock), 0,
+                       stream, alpha, fnx, fny, fnz, ng, tip, total, extra);
     return hipGetLastError();
 }
 

@@ -168,6 +168,16 @@ int pf_qoi_volume(pf_ctx* ctx, int max_id, int* counts, int* first, long long* n
  * sorted because the reference's order comes from unordered_map iteration. */
 int pf_qoi_junctions3d(pf_ctx* ctx, int* rows, int cap, int* count);
 
+/* QOI::calculateLineQoIs (3DgrainMPI/QOI.cpp:38-95) without its loss-area
+ * term and printout: *new_tip = the tip plane after cur_tip (the first plane
+ * above cur_tip whose interior holds a liquid point, minus one; fnz - 2 if no
+ * plane up to fnz - 2 holds liquid, where the reference would read past the
+ * field), total[g] / extra[g] = interior points with alpha = g + 1 in all
+ * planes / in planes above *new_tip, for g < num_theta (the reference's
+ * out-of-range writes for larger ids are skipped).  The caller adds the
+ * moving domain's loss area to total. */
+int pf_qoi_line(pf_ctx* ctx, int cur_tip, int* new_tip, int* total, int* extra);
+
 /* Liquid-start re-initialisation (3DgrainMPI/APTPhaseField.cu:809-818): both
  * buffers set to -1, then slot 0 = (initial phi0, collected alpha) where
  * phi0 > -0.995.  Follows a pf_collect with PF_COLLECT_CLEAR_ALPHA. */

@@ -1,6 +1,7 @@
 """QoI restatement (numpy) -- TEST INFRASTRUCTURE ONLY.
 
-Follows 3DgrainMPI/QOI.cpp: per-grain volume over interior points with
+Follows 3DgrainMPI/QOI.cpp: line QoIs (tip scan, per-grain areas, :38-95);
+per-grain volume over interior points with
 alpha > 0 in first-seen (k, j, i) scan order (:245-278); 3-D junctions: solid
 interior points whose 19-point neighbourhood holds no liquid and exactly 4
 distinct ids with the largest count <= 10 (:280-358); ids compared as sorted
@@ -16,6 +17,29 @@ def volume(alpha, fnx, fny, fnz):
     order = np.argsort(first)
     counts = np.bincount(np.searchsorted(ids, a), minlength=ids.size)
     return ids[order].tolist(), counts[order].tolist()
+
+
+def line_qois(alpha, fnx, fny, fnz, cur_tip, num_grains):
+    """QOI::calculateLineQoIs (3DgrainMPI/QOI.cpp:38-95) without the loss-area
+    term: step the tip up to the first plane whose interior holds liquid
+    (alpha 0), minus one (:47-67; bounded at fnz - 2), then per grain id
+    1..num_grains the interior points in all planes (total) and above the tip
+    (extra) (:71-90)."""
+    A = np.asarray(alpha).reshape(fnz, fny, fnx)
+    core = A[:, 1:-1, 1:-1]
+    tip = cur_tip
+    while True:
+        tip += 1
+        if tip >= fnz - 1 or (core[tip] == 0).any():
+            break
+    tip -= 1
+    inner = A[1:-1, 1:-1, 1:-1]
+    ks = np.arange(1, fnz - 1)[:, None, None] * np.ones_like(inner)
+    ok = (inner > 0) & (inner <= num_grains)
+    total = np.bincount(inner[ok] - 1, minlength=num_grains)[:num_grains]
+    up = ok & (ks > tip)
+    extra = np.bincount(inner[up] - 1, minlength=num_grains)[:num_grains]
+    return tip, total, extra
 
 
 def junctions3d(alpha, fnx, fny, fnz):

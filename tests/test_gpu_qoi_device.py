@@ -62,3 +62,21 @@ def test_device_qoi_on_block_pattern():
     assert above20 == int((a > 20).sum())
     keep = [(g, n) for g, n in zip(ref_ids, ref_counts) if g <= 20]
     assert list(zip(ids20, cnt20)) == keep
+
+
+def test_device_line_qois():
+    c = cases.make_line_case("qoi-l", n=(48, 40, 40), grains=30, K=8, seed=33, G=1.0, R=0.1,
+                             z0_cells=12)
+    ctx = gpu_context(c)
+    p = c.params
+    dims = (int(p["fnx"]), int(p["fny"]), int(p["fnz"]))
+    ng = int(p["num_theta"])
+    for s0 in (1, 31):
+        ctx.step(30, s0)
+        _, alpha = ctx.collect(0, capi.PF_COLLECT_CLEAR_ALPHA)
+        ctx.collect_on_device(0, capi.PF_COLLECT_CLEAR_ALPHA)
+        for cur in (0, 5):
+            tip, total, extra = ctx.qoi_line(cur)
+            rt, rtot, rext = qoi.line_qois(alpha, *dims, cur, ng)
+            assert tip == rt
+            assert np.array_equal(total, rtot) and np.array_equal(extra, rext)

@@ -563,3 +563,43 @@ def test_float_thresholds_equal_double_compares():
     xd = x.astype(np.float64)
     assert np.array_equal(xd > -0.995, x > np.float32(-0.995))
     assert np.array_equal(xd < -0.9999, x < np.float32(-0.9999))
+
+
+def test_line_qois_restatement_against_loops_and_a_hand_case():
+    """oracle/qoi.py::line_qois against a literal transcription of the loops of
+    3DgrainMPI/QOI.cpp:47-90 on random fields, and a hand-checked field."""
+    from oracle import qoi
+    rng = np.random.default_rng(0)
+    fnx, fny, fnz = 7, 6, 9
+    for trial in range(6):
+        A = rng.integers(0, 6, size=(fnz, fny, fnx)).astype(np.int32)
+        A[1:1 + trial % 5, 1:-1, 1:-1] = rng.integers(1, 6, size=(trial % 5, fny - 2, fnx - 2))
+        for cur in (0, 1, 2, 5):
+            tip = cur
+            while True:
+                tip += 1
+                if tip >= fnz - 1:
+                    break
+                if any(A[tip, j, i] == 0 for j in range(1, fny - 1) for i in range(1, fnx - 1)):
+                    break
+            tip -= 1
+            tot, ext = [0] * 4, [0] * 4
+            for k in range(1, fnz - 1):
+                for j in range(1, fny - 1):
+                    for i in range(1, fnx - 1):
+                        a = A[k, j, i]
+                        if 0 < a <= 4:
+                            tot[a - 1] += 1
+                            if k > tip:
+                                ext[a - 1] += 1
+            t2, tot2, ext2 = qoi.line_qois(A.ravel(), fnx, fny, fnz, cur, 4)
+            assert t2 == tip and list(tot2) == tot and list(ext2) == ext
+    # hand case: 3x3 interior, planes 1-2 solid grain 1, plane 3 grain 2 with
+    # one liquid point, plane 4 liquid: the tip stops at plane 3 - 1 = 2;
+    # grain 1 has 18 points (none above the tip), grain 2 has 8 (all above)
+    H = np.zeros((6, 5, 5), dtype=np.int32)
+    H[1:3, 1:4, 1:4] = 1
+    H[3, 1:4, 1:4] = 2
+    H[3, 2, 2] = 0
+    tip, total, extra = qoi.line_qois(H.ravel(), 5, 5, 6, 0, 2)
+    assert tip == 2 and list(total) == [18, 8] and list(extra) == [0, 8]
