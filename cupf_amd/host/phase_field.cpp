@@ -368,7 +368,12 @@ std::string PhaseField::outputStem() const
 void PhaseField::OutputQoIs()
 {
     const DesignSettingData* ds = GetSetDesignSetting();
-    qois->searchJunctionsOnImage(params, ds->useLineConfig ? alpha_i_full.data() : alpha.data());
+    if (ds->useLineConfig)
+        qois->searchJunctionsOnImage(params, alpha_i_full.data());
+    else if (deviceContext())  // the device copy of the last collected alpha: only the junctions cross PCIe
+        static_cast<QOI3D*>(qois)->searchJunctionsDevice(deviceContext(), params);
+    else
+        qois->searchJunctionsOnImage(params, alpha.data());
     const std::string name = outputStem() + "_walltime" + std::to_string(params.walltime) + "_frames" +
                              std::to_string(params.nts) + "_Mt" + std::to_string(params.Mt) + "_rank" +
                              std::to_string(GetMPIManager()->rank) + ".h5";
@@ -555,8 +560,13 @@ void APTPhaseField::evolve()
             }
             if ((2 * kt + 2) % TIPP == 0) moveDomain(&md);
         } else if ((2 * kt + 2) % qoikts == 0) {
-            check(pf_collect(ctx_, 0, 0, nullptr, alpha.data()), "pf_collect");
-            qois->calculateQoIs(params, alpha.data(), (2 * kt + 2) / qoikts);
+            // per-grain volumes on the device; only the counts cross PCIe.  Ids
+            // above num_theta (nucleated grains) take the host path.
+            check(pf_collect(ctx_, 0, 0, nullptr, nullptr), "pf_collect");
+            if (!qois->calculateQoIsDevice(ctx_, params)) {
+                check(pf_collect(ctx_, 0, 0, nullptr, alpha.data()), "pf_collect");
+                qois->calculateQoIs(params, alpha.data(), (2 * kt + 2) / qoikts);
+            }
         }
         if ((2 * kt + 2) % fieldkts == 0) {
             check(pf_collect(ctx_, 0, PF_COLLECT_CLEAR_ALPHA, nullptr, alpha.data()), "pf_collect");

@@ -47,6 +47,8 @@ public:
     virtual void evolve();
     void OutputQoIs();
     void OutputField(int currentStep);
+    // The GPU context holding the last collected alpha (none for the base class).
+    virtual pf_ctx* deviceContext() const { return nullptr; }
     // Derived constants (params.txt, `name value` per line) and the initial
     // phi/alpha (raw float32/int32) of initField, for checking the set-up.
     void dumpSetup(const std::string& dir) const;
@@ -75,6 +77,7 @@ public:
     void cudaSetup() override;
     void evolve() override;
     pf_ctx* context() const { return ctx_; }
+    pf_ctx* deviceContext() const override { return ctx_; }
 
 protected:
     virtual void moveDomain(MovingDomain* movingDomainManager);

@@ -6,7 +6,11 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <stdexcept>
+#include <string>
 #include <unordered_map>
+
+#include "../../include/pf_capi.h"
 
 namespace {
 
@@ -101,6 +105,31 @@ void QOI::calculateQoIs(const GlobalConstants& params, const int* alpha, int)
             }
 }
 
+bool QOI::calculateQoIsDevice(pf_ctx* ctx, const GlobalConstants& params)
+{
+    const int max_id = std::max(1, params.num_theta);
+    std::vector<int> counts((size_t)max_id + 1), first((size_t)max_id + 1);
+    long long above = 0;
+    if (pf_qoi_volume(ctx, max_id, counts.data(), first.data(), &above) != PF_OK)
+        throw std::runtime_error(std::string("pf_qoi_volume: ") + pf_last_error());
+    if (above > 0) return false;
+    std::vector<int> ids;
+    for (int g = 1; g <= max_id; ++g)
+        if (counts[g] > 0) ids.push_back(g);
+    // first-seen order of the host scan (3DgrainMPI/QOI.cpp:245-278)
+    std::sort(ids.begin(), ids.end(), [&](int a, int b) { return first[a] < first[b]; });
+    auto& volume = mQoIVectorIntData["volume"];
+    auto& g2pf = mQoIVectorIntData["grainToPF"];
+    volume.clear();
+    g2pf.clear();
+    for (int g : ids) {
+        g2pf.push_back(g);
+        volume.push_back(counts[g]);
+    }
+    mNumActiveGrains = (int)ids.size();
+    return true;
+}
+
 void QOI::sampleHeights(int& cur_tip, const int* alpha, int fnx, int fny, int fnz)
 {
     cur_tip = scan_tip(cur_tip, alpha, fnx, fny, fnz);
@@ -182,4 +211,18 @@ void QOI3D::searchJunctionsOnImage(const GlobalConstants& params, const int* alp
                     ++node_cnt;
                 }
             }
+}
+
+void QOI3D::searchJunctionsDevice(pf_ctx* ctx, const GlobalConstants&)
+{
+    auto& nodes = mQoIVectorIntData["node_region"];
+    nodes.assign((size_t)40 * mNumActiveGrains * mNumNodeFeatures, -1);
+    const int cap = (int)(nodes.size() / mNumNodeFeatures);
+    std::vector<int> rows((size_t)8 * std::max(cap, 1));
+    int n = 0;
+    if (pf_qoi_junctions3d(ctx, rows.data(), cap, &n) != PF_OK)
+        throw std::runtime_error(std::string("pf_qoi_junctions3d: ") + pf_last_error());
+    // rows come in scan order; the host search keeps the first `cap` of them
+    for (int q = 0; q < std::min(n, cap); ++q)
+        for (int u = 0; u < 8; ++u) nodes[(size_t)q * mNumNodeFeatures + u] = rows[(size_t)8 * q + u];
 }

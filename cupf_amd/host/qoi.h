@@ -9,6 +9,8 @@
 
 #include "params.h"
 
+struct pf_ctx;
+
 class QOI {
 public:
     virtual ~QOI() = default;
@@ -16,6 +18,10 @@ public:
     void calculateLineQoIs(const GlobalConstants& params, int& cur_tip, const int* alpha, int kt,
                            const float* z, const int* loss_area, int move_count);
     void calculateQoIs(const GlobalConstants& params, const int* alpha, int kt);
+    // calculateQoIs on the context's device alpha (the last pf_collect);
+    // false (nothing changed) if some id exceeds num_theta (nucleated grains):
+    // the caller then downloads alpha and uses calculateQoIs.
+    bool calculateQoIsDevice(pf_ctx* ctx, const GlobalConstants& params);
     void sampleHeights(int& cur_tip, const int* alpha, int fnx, int fny, int fnz);
     std::map<std::string, std::vector<int>> mQoIVectorIntData;
     std::map<std::string, std::vector<float>> mQoIVectorFloatData;
@@ -35,6 +41,8 @@ class QOI3D : public QOI {
 public:
     explicit QOI3D(const GlobalConstants& params);
     void searchJunctionsOnImage(const GlobalConstants& params, const int* alpha) override;
+    // the same search on the context's device alpha (the last pf_collect)
+    void searchJunctionsDevice(pf_ctx* ctx, const GlobalConstants& params);
 
 private:
     int mNumNodeFeatures = 9;
