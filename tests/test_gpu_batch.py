@@ -24,10 +24,10 @@ SPECS = [
 ]
 
 
-def _case(spec, K=8):
+def _case(spec, K=8, g27=0):
     n, grains, seed, G, R, bc, tt, z0c = spec
     return cases.make_line_case(f"b{seed}", n=n, grains=grains, K=K, seed=seed, G=G, R=R,
-                                z0_cells=z0c, bc=bc, thermal_type=tt)
+                                z0_cells=z0c, bc=bc, thermal_type=tt, gather27=g27)
 
 
 def _same(c1, c2, which=(0, 1)):
@@ -39,8 +39,8 @@ def _same(c1, c2, which=(0, 1)):
     return True
 
 
-def _pairs(K=8):
-    cs = [_case(sp, K) for sp in SPECS]
+def _pairs(K=8, g27=0):
+    cs = [_case(sp, K, g27) for sp in SPECS]
     return cs, [gpu_context(c) for c in cs], [gpu_context(c) for c in cs]
 
 
@@ -99,6 +99,25 @@ def test_batch_autotune_and_chunk_depths_do_not_change_results():
     b.close()
 
 
+@pytest.mark.parametrize("K,g27", [(5, 0), (5, 1), (8, 1)])
+def test_batch_other_variants_match_single_steps(K, g27):
+    """The K=5 slot set and the 27-point id gather (3Dgrain/) batched: eager
+    steps, graph replays and a changed chunk depth, bit for bit against the
+    same members stepped alone."""
+    cs, batched, single = _pairs(K, g27)
+    b = capi.Batch(batched)
+    b.step(10)
+    b.step(16, capi.PF_BATCH_GRAPH)
+    b.kchunk = 8
+    b.step(6)
+    for c in single:
+        c.step(32, 1)
+    for i, (x, y) in enumerate(zip(batched, single)):
+        assert _same(x, y), f"member {i} differs (K={K}, 27-point gather={g27})"
+    assert [m.overflow_count() for m in batched] == [m.overflow_count() for m in single]
+    b.close()
+
+
 def test_batch_member_against_oracle():
     cs = [_case(sp) for sp in SPECS[:3]]
     ctxs = [gpu_context(c) for c in cs]
@@ -113,11 +132,14 @@ def test_batch_member_against_oracle():
     b.close()
 
 
-def test_batch_rejects_mixed_k_and_nucleation():
+def test_batch_rejects_mixed_k_gather_and_nucleation():
     c8 = gpu_context(_case(SPECS[0], K=8))
     c5 = gpu_context(_case(SPECS[1], K=5))
     with pytest.raises(capi.PFError):
         capi.Batch([c8, c5])
+    g27 = gpu_context(_case(SPECS[1], K=8, g27=1))
+    with pytest.raises(capi.PFError):
+        capi.Batch([c8, g27])
     b = capi.Batch([c8])
     with pytest.raises(capi.PFError):
         b.step(2, capi.PF_STEP_NUCLEATE)
