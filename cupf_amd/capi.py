@@ -261,7 +261,7 @@ class APTContext:
         _check(load().pf_collect(self._h, which, flags, None, None))
 
     def qoi_volume(self, max_id: int):
-        """Device QOI3D::calculateQoIs on the collected alpha: (ids in
+        """Device QOI::calculateQoIs on the collected alpha: (ids in
         first-seen scan order, their point counts, points with an id above
         max_id)."""
         counts = np.zeros(max_id + 1, dtype=np.int32)

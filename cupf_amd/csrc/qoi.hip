@@ -1,7 +1,7 @@
 // qoi.hip -- device-side quantities of interest on the collected grain-id
 // field (alpha, fnx*fny*fnz with its halo, as pf_collect leaves it): per-grain
-// volumes (QOI3D::calculateQoIs, 3DgrainMPI/QOI.cpp:245-278), the 3-D
-// junction search (QOI3D::searchJunctionsOnImage, 3DgrainMPI/QOI.cpp:280-358)
+// volumes (QOI::calculateQoIs, 3DgrainMPI/QOI.cpp:198-231), the 3-D
+// junction search (QOI3D::searchJunctionsOnImage, 3DgrainMPI/QOI.cpp:233-311)
 // and the line QoIs' tip scan and grain areas (QOI::calculateLineQoIs,
 // 3DgrainMPI/QOI.cpp:38-95).  Only the results cross PCIe, not the field.
 #include "apt_device.h"
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void qoi_liquid_planes_kernel(const int* __
     }
 }
 
-// Line QoIs (QOI::calculateLineQoIs, 3DgrainMPI/QOI.cpp:71-95): per grain id
+// Line QoIs (QOI::calculateLineQoIs, 3DgrainMPI/QOI.cpp:71-90): per grain id
 // 1..ng, interior points (total) and those above the tip plane (extra).
 __global__ __launch_bounds__(kBlock) void qoi_area_kernel(const int* __restrict__ alpha, int fnx,
                                                           int fny, int fnz, int ng, int tip,

@@ -15,7 +15,7 @@
 namespace {
 
 // First plane above cur_tip whose interior holds a liquid point (alpha == 0),
-// minus one (3DgrainMPI/QOI.cpp:94-114,162-185).
+// minus one (3DgrainMPI/QOI.cpp:47-67,162-185).
 int scan_tip(int cur_tip, const int* alpha, int fnx, int fny, int fnz)
 {
     for (;;) {
@@ -59,6 +59,7 @@ QOI3D::QOI3D(const GlobalConstants&)
     mQoIVectorFloatData.emplace("geodesic_y_coors", std::vector<float>());
 }
 
+// 3DgrainMPI/QOI.cpp:38-97
 void QOI::calculateLineQoIs(const GlobalConstants& params, int& cur_tip, const int* alpha, int kt,
                             const float* z, const int* loss_area, int move_count)
 {
@@ -81,6 +82,7 @@ void QOI::calculateLineQoIs(const GlobalConstants& params, int& cur_tip, const i
     for (int g = 0; g < NG; ++g) total[(size_t)kt * NG + g] += loss_area[g];
 }
 
+// 3DgrainMPI/QOI.cpp:198-231
 void QOI::calculateQoIs(const GlobalConstants& params, const int* alpha, int)
 {
     auto& volume = mQoIVectorIntData["volume"];
@@ -116,7 +118,7 @@ bool QOI::calculateQoIsDevice(pf_ctx* ctx, const GlobalConstants& params)
     std::vector<int> ids;
     for (int g = 1; g <= max_id; ++g)
         if (counts[g] > 0) ids.push_back(g);
-    // first-seen order of the host scan (3DgrainMPI/QOI.cpp:245-278)
+    // first-seen order of the host scan (3DgrainMPI/QOI.cpp:198-231)
     std::sort(ids.begin(), ids.end(), [&](int a, int b) { return first[a] < first[b]; });
     auto& volume = mQoIVectorIntData["volume"];
     auto& g2pf = mQoIVectorIntData["grainToPF"];
@@ -130,11 +132,13 @@ bool QOI::calculateQoIsDevice(pf_ctx* ctx, const GlobalConstants& params)
     return true;
 }
 
+// 3DgrainMPI/QOI.cpp:162-185
 void QOI::sampleHeights(int& cur_tip, const int* alpha, int fnx, int fny, int fnz)
 {
     cur_tip = scan_tip(cur_tip, alpha, fnx, fny, fnz);
 }
 
+// 3DgrainMPI/QOI.cpp:103-160
 void QOILine::searchJunctionsOnImage(const GlobalConstants& params, const int* alpha)
 {
     const int fnx = params.fnx, fny = params.fny;
@@ -173,6 +177,7 @@ void QOILine::searchJunctionsOnImage(const GlobalConstants& params, const int* a
     }
 }
 
+// 3DgrainMPI/QOI.cpp:233-311
 void QOI3D::searchJunctionsOnImage(const GlobalConstants& params, const int* alpha)
 {
     auto& nodes = mQoIVectorIntData["node_region"];

@@ -1,5 +1,5 @@
 // qoi.h -- host quantities of interest computed from the grain-id field alpha
-// (3DgrainMPI/QOI.h:12-48): per-grain volume, quadruple-junction search,
+// (3DgrainMPI/QOI.h:12-47): per-grain volume, quadruple-junction search,
 // line-mode tip height / areas / cross-sections, moving-window tip samples.
 #pragma once
 

@@ -152,7 +152,7 @@ int pf_collect(pf_ctx* ctx, int which, int flags, float* phi_out, int* alpha_out
 /* Device-side QoIs on the context's alpha field as the last pf_collect left
  * it; only the results are copied to the host.
  *
- * QOI3D::calculateQoIs (3DgrainMPI/QOI.cpp:245-278): for grain ids
+ * QOI::calculateQoIs (3DgrainMPI/QOI.cpp:198-231): for grain ids
  * 1..max_id, counts[id] = interior points with that alpha and first[id] =
  * scan index (k-major, then j, then i, from 0) of the first such point, or
  * INT_MAX if none; ordering the ids with a count by first[] gives the
@@ -160,7 +160,7 @@ int pf_collect(pf_ctx* ctx, int which, int flags, float* phi_out, int* alpha_out
  * *n_above = interior points whose alpha exceeds max_id (nucleated ids). */
 int pf_qoi_volume(pf_ctx* ctx, int max_id, int* counts, int* first, long long* n_above);
 
-/* QOI3D::searchJunctionsOnImage (3DgrainMPI/QOI.cpp:280-358): solid interior
+/* QOI3D::searchJunctionsOnImage (3DgrainMPI/QOI.cpp:233-311): solid interior
  * points whose 19-point neighbourhood holds no liquid and exactly 4 distinct
  * ids, none more than 10 times.  Up to cap rows of 8 ints (i, j, k, largest
  * count, the 4 ids ascending) in the reference's scan order (k, then j, then

@@ -2,9 +2,9 @@
 
 Follows 3DgrainMPI/QOI.cpp: line QoIs (tip scan, per-grain areas, :38-95);
 per-grain volume over interior points with
-alpha > 0 in first-seen (k, j, i) scan order (:245-278); 3-D junctions: solid
+alpha > 0 in first-seen (k, j, i) scan order (:198-231); 3-D junctions: solid
 interior points whose 19-point neighbourhood holds no liquid and exactly 4
-distinct ids with the largest count <= 10 (:280-358); ids compared as sorted
+distinct ids with the largest count <= 10 (:233-311); ids compared as sorted
 sets (the reference's order comes from unordered_map iteration).
 """
 import numpy as np

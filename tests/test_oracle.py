@@ -603,3 +603,86 @@ def test_line_qois_restatement_against_loops_and_a_hand_case():
     H[3, 2, 2] = 0
     tip, total, extra = qoi.line_qois(H.ravel(), 5, 5, 6, 0, 2)
     assert tip == 2 and list(total) == [18, 8] and list(extra) == [0, 8]
+
+
+def _volume_loops(A):
+    """Literal transcription of QOI::calculateQoIs (3DgrainMPI/QOI.cpp:198-231)."""
+    fnz, fny, fnx = A.shape
+    index, ids, vol = {}, [], []
+    for k in range(1, fnz - 1):
+        for j in range(1, fny - 1):
+            for i in range(1, fnx - 1):
+                a = int(A[k, j, i])
+                if a > 0:
+                    if a not in index:
+                        index[a] = len(ids)
+                        ids.append(a)
+                        vol.append(0)
+                    vol[index[a]] += 1
+    return ids, vol
+
+
+def _junction_loops(A):
+    """Literal transcription of QOI3D::searchJunctionsOnImage
+    (3DgrainMPI/QOI.cpp:233-311); ids as a sorted tuple."""
+    fnz, fny, fnx = A.shape
+    out = []
+    for k in range(1, fnz - 1):
+        for j in range(1, fny - 1):
+            for i in range(1, fnx - 1):
+                if A[k, j, i] == 0:
+                    continue
+                occur, zero = {}, 0
+                for dk in (-1, 0, 1):
+                    for dj in (-1, 0, 1):
+                        for di in (-1, 0, 1):
+                            if di * di + dj * dj + dk * dk > 2:
+                                continue
+                            a = int(A[k + dk, j + dj, i + di])
+                            zero += a == 0
+                            occur[a] = occur.get(a, 0) + 1
+                if zero > 0:
+                    continue
+                if len(occur) == 4 and max(occur.values()) <= 10:
+                    out.append((i, j, k, max(occur.values()), tuple(sorted(occur))))
+    return out
+
+
+def test_volume_and_junction_restatements_against_loops_and_a_hand_case():
+    """oracle/qoi.py::volume and ::junctions3d against the loops above on
+    random block fields (many 4-id neighbourhoods, some liquid, counts on
+    both sides of the <= 10 bound), and a hand-worked corner of 4 grains."""
+    from oracle import qoi
+    rng = np.random.default_rng(1)
+    seen_big = seen_junction = 0
+    for trial in range(8):
+        fnx, fny, fnz = 9 + trial % 3, 8, 7 + trial % 2
+        k, j, i = np.meshgrid(np.arange(fnz), np.arange(fny), np.arange(fnx), indexing="ij")
+        b = 2 + trial % 2
+        table = rng.integers(1, 9, size=(fnz // b + 1, fny // b + 1, fnx // b + 1))
+        A = table[k // b, j // b, i // b].astype(np.int32)
+        A[rng.random(A.shape) < 0.02] = 0
+        ids, vol = _volume_loops(A)
+        assert qoi.volume(A.ravel(), fnx, fny, fnz) == (ids, vol)
+        rows = _junction_loops(A)
+        assert qoi.junctions3d(A.ravel(), fnx, fny, fnz) == rows
+        seen_junction += len(rows)
+        seen_big += sum(r[3] == 10 for r in rows)
+    assert seen_junction > 20 and seen_big > 0
+    # hand case: interior 4^3, four quadrants in (i, j) split between 2 and 3,
+    # the same in every plane.  The 19-point neighbourhood of (2, 2, k) holds
+    # id 1 ten times, ids 2 and 3 four times, id 4 once (the dk = 0 diagonal):
+    # the four points (2|3, 2|3, k) of each plane are junctions.  A liquid
+    # point at (2, 2, 1) removes every point whose neighbourhood holds it:
+    # the four of plane 1 and (2, 2, 2), (3, 2, 2), (2, 3, 2); (3, 3, 2) is
+    # at distance^2 3 and stays.
+    k, j, i = np.meshgrid(np.arange(6), np.arange(6), np.arange(6), indexing="ij")
+    H = (1 + (i >= 3) + 2 * (j >= 3)).astype(np.int32)
+    full = qoi.junctions3d(H.ravel(), 6, 6, 6)
+    assert full == [(ii, jj, kk, 10, (1, 2, 3, 4)) for kk in range(1, 5) for jj in (2, 3)
+                    for ii in (2, 3)]
+    H[1, 2, 2] = 0
+    want = [(3, 3, 2, 10, (1, 2, 3, 4))] + [(ii, jj, kk, 10, (1, 2, 3, 4)) for kk in (3, 4)
+                                            for jj in (2, 3) for ii in (2, 3)]
+    assert qoi.junctions3d(H.ravel(), 6, 6, 6) == want
+    assert qoi.volume(H.ravel(), 6, 6, 6) == ([1, 2, 3, 4], [15, 16, 16, 16])
