@@ -1223,7 +1223,7 @@ int pf_batch_step(pf_batch* b, int nsteps, const int* first_step, int flags)
     // candidates are timed in two rounds, the second in reverse order, so a
     // cost that drifts steadily over the tuning steps weighs on all alike.
     constexpr int kTuneEvery = 400, kTuneSteps = 4;
-    const int cand[] = {4, 8, 16, 24};
+    const int cand[] = {2, 4, 8, 16};
     constexpr int nc = (int)(sizeof(cand) / sizeof(cand[0]));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(hipEventCreate(&e0));

@@ -233,7 +233,7 @@ typedef struct pf_batch pf_batch;
 /* pf_batch_step flags (beside PF_STEP_INITIAL_UC) */
 #define PF_BATCH_GRAPH 4      /* replay groups of 8 steps from one captured hipGraph */
 #define PF_BATCH_AUTOTUNE 8   /* every 400 steps, time 4 real steps with each z-chunk
-                                 depth 4, 8, 16, 24, twice (forward, then reverse
+                                 depth 2, 4, 8, 16, twice (forward, then reverse
                                  order), and keep the fastest for the next 400 steps
                                  (a tuning round needs >= 56 steps left) */
 

@@ -89,7 +89,7 @@ def test_batch_autotune_and_chunk_depths_do_not_change_results():
     cs, batched, single = _pairs()
     b = capi.Batch(batched)
     b.step(80, capi.PF_BATCH_AUTOTUNE)
-    assert b.kchunk in (4, 8, 16, 24)
+    assert b.kchunk in (2, 4, 8, 16)
     b.kchunk = 3
     b.step(7)
     for c in single:
