@@ -152,6 +152,48 @@ def serving_block(count: int, K: int) -> dict:
     }
 
 
+def qoi_block(ctx, p) -> dict:
+    """The device QoIs on the current state (median of 5 calls each, wall
+    clock around the C-ABI call, which allocates its small result buffers,
+    launches, copies the results back and synchronises), beside the download
+    of the collected fields that a host-side QoI pass needs first."""
+    from cupf_amd import capi
+
+    def med(fn, reps=5):
+        ts = []
+        for _ in range(reps):
+            ctx.sync()
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        return sorted(ts)[reps // 2] * 1e3
+    ms_collect = med(lambda: ctx.collect_on_device(0, capi.PF_COLLECT_CLEAR_ALPHA))
+    ms_volume = med(lambda: ctx.qoi_volume(int(p["num_theta"])))
+    ms_junc = med(ctx.qoi_junctions3d)
+    ms_line = med(lambda: ctx.qoi_line(0))
+    ms_host = med(lambda: ctx.collect(0, capi.PF_COLLECT_CLEAR_ALPHA), 3)
+    ids, _, above = ctx.qoi_volume(int(p["num_theta"]))
+    nj = len(ctx.qoi_junctions3d())
+    field = int(p["fnx"]) * int(p["fny"]) * int(p["fnz"]) * 4
+    log(f"[bench] device QoIs: collect {ms_collect:.3f} ms, volume {ms_volume:.3f} ms ({len(ids)} grains), "
+        f"junctions {ms_junc:.3f} ms ({nj}), line {ms_line:.3f} ms; "
+        f"collect + download of phi and alpha {ms_host:.1f} ms")
+    return {
+        "workload": "QoIs of the 256x256x512 state after the timed steps",
+        "collect_on_device_ms": round(ms_collect, 3),
+        "volume_ms": round(ms_volume, 3),
+        "junctions3d_ms": round(ms_junc, 3),
+        "line_ms": round(ms_line, 3),
+        "collect_and_download_ms": round(ms_host, 2),
+        "alpha_bytes": field,
+        "grains": len(ids),
+        "ids_above_num_theta": above,
+        "junctions": nj,
+        "note": "device calls each include their result allocation, launch, copy-back and sync; "
+                "a host QoI pass would first need the download",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +302,8 @@ def main() -> int:
         }
         log(f"[bench] cpu oracle: {cpu['value']} Mcell-steps/s on {threads} threads")
 
+    qoi = qoi_block(ctx, p)
+
     ens = None
     if not args.no_ensemble:
         ens = ensemble_block(args.ensemble_sims, args.K)
@@ -312,6 +356,7 @@ def main() -> int:
             "dense_equiv_gbs": round(dense_equiv, 1),
         },
         "cpu_baseline": cpu,
+        "qoi": qoi,
         "ensemble": ens,
         "serving": srv,
     }

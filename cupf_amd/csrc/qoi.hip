@@ -10,33 +10,53 @@ namespace cupf {
 
 namespace {
 
+// Interior point s (scan order: k-major, then j, then i) to its field index.
+// The context keeps K*fnx*fny*fnz below 2^31, so 32-bit arithmetic holds.
+__device__ __forceinline__ int interior_index(unsigned s, int nx, int ny, int fnx, int fny, int* k_out)
+{
+    const unsigned i = s % (unsigned)nx, r = s / (unsigned)nx;
+    const unsigned j = r % (unsigned)ny, k = r / (unsigned)ny;
+    *k_out = (int)k + 1;
+    return ((int)k + 1) * fnx * fny + ((int)j + 1) * fnx + (int)i + 1;
+}
+
 // Per-grain point counts and first occurrence over interior points: counts[a]
 // += 1 and first[a] = min scan index for 1 <= a <= max_id, where the scan
 // index runs k-major, then j, then i (the reference's loop order, so sorting
 // ids by first[] reproduces its first-seen order).  Points with a larger id
-// are counted in *above.
+// are counted in *above.  The 64 points of a wave are consecutive in scan
+// order and usually hold one or two ids, so the lanes of one id add their
+// count with a single atomic (per-point atomics on the few ids of a thin
+// solid layer serialise on those addresses); the lowest such lane holds the
+// smallest scan index.
 __global__ __launch_bounds__(kBlock) void qoi_volume_kernel(const int* __restrict__ alpha, int fnx,
                                                             int fny, int fnz, int max_id,
                                                             int* counts, int* first,
                                                             unsigned long long* above)
 {
     const int nx = fnx - 2, ny = fny - 2, nz = fnz - 2;
-    const long long n = (long long)nx * ny * nz;
+    const unsigned n = (unsigned)nx * ny * nz;
+    const int lane = threadIdx.x & 63;
     unsigned long long over = 0ull;
-    for (long long s = (long long)blockIdx.x * kBlock + threadIdx.x; s < n;
-         s += (long long)gridDim.x * kBlock) {
-        const int i = 1 + (int)(s % nx);
-        const long long r = s / nx;
-        const int j = 1 + (int)(r % ny);
-        const int k = 1 + (int)(r / ny);
-        const int a = alpha[(long long)k * fnx * fny + (long long)j * fnx + i];
-        if (a <= 0) continue;
-        if (a > max_id) {
-            ++over;
-            continue;
+    // wave-uniform trip count: every lane reaches the ballots
+    for (unsigned base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < n;
+         base += gridDim.x * kBlock) {
+        const unsigned s = base + lane;
+        int a = 0, k;
+        if (s < n) a = alpha[interior_index(s, nx, ny, fnx, fny, &k)];
+        over += a > max_id ? 1ull : 0ull;
+        const bool valid = a > 0 && a <= max_id;
+        unsigned long long pend = __ballot(valid);
+        while (pend) {
+            const int leader = __ffsll((long long)pend) - 1;
+            const int id = __shfl(a, leader);
+            const unsigned long long same = __ballot(valid && a == id) & pend;
+            if (lane == leader) {
+                atomicAdd(&counts[id], (int)__popcll(same));
+                atomicMin(&first[id], (int)s);
+            }
+            pend &= ~same;
         }
-        atomicAdd(&counts[a], 1);
-        atomicMin(&first[a], (int)s);
     }
     if (over) atomicAdd(above, over);
 }
@@ -51,15 +71,13 @@ __global__ __launch_bounds__(kBlock) void qoi_junctions_kernel(const int* __rest
                                                                unsigned* count)
 {
     const int nx = fnx - 2, ny = fny - 2, nz = fnz - 2;
-    const long long n = (long long)nx * ny * nz;
-    const long long A = (long long)fnx * fny;
-    for (long long s = (long long)blockIdx.x * kBlock + threadIdx.x; s < n;
-         s += (long long)gridDim.x * kBlock) {
-        const int i = 1 + (int)(s % nx);
-        const long long r = s / nx;
-        const int j = 1 + (int)(r % ny);
-        const int k = 1 + (int)(r / ny);
-        const long long C = k * A + (long long)j * fnx + i;
+    const unsigned n = (unsigned)nx * ny * nz;
+    const int A = fnx * fny;
+    for (unsigned s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock) {
+        const unsigned r = s / (unsigned)nx;
+        const int i = 1 + (int)(s - r * (unsigned)nx);
+        const int j = 1 + (int)(r % (unsigned)ny), k = 1 + (int)(r / (unsigned)ny);
+        const int C = k * A + j * fnx + i;
         if (alpha[C] == 0) continue;
         int id[4] = {0, 0, 0, 0}, cnt[4] = {0, 0, 0, 0};
         int nd = 0;
@@ -71,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void qoi_junctions_kernel(const int* __rest
 #pragma unroll
                 for (int di = -1; di <= 1; ++di) {
                     if (di * di + dj * dj + dk * dk > 2) continue;
-                    const int a = alpha[C + dk * A + (long long)dj * fnx + di];
+                    const int a = alpha[C + dk * A + dj * fnx + di];
                     ok = ok && a != 0;
                     bool hit = false;
 #pragma unroll
@@ -123,41 +141,61 @@ __global__ __launch_bounds__(kBlock) void qoi_junctions_kernel(const int* __rest
 }
 
 // Planes whose interior holds a liquid point (alpha 0): liq[k] = 1.  Every
-// writer stores the same value, so plain stores suffice.
+// writer stores the same value, so plain stores suffice; one lane per plane
+// present in the wave stores.
 __global__ __launch_bounds__(kBlock) void qoi_liquid_planes_kernel(const int* __restrict__ alpha,
                                                                    int fnx, int fny, int fnz,
                                                                    unsigned* liq)
 {
     const int nx = fnx - 2, ny = fny - 2, nz = fnz - 2;
-    const long long n = (long long)nx * ny * nz;
-    for (long long s = (long long)blockIdx.x * kBlock + threadIdx.x; s < n;
-         s += (long long)gridDim.x * kBlock) {
-        const int i = 1 + (int)(s % nx);
-        const long long r = s / nx;
-        const int j = 1 + (int)(r % ny);
-        const int k = 1 + (int)(r / ny);
-        if (alpha[(long long)k * fnx * fny + (long long)j * fnx + i] == 0) liq[k] = 1u;
+    const unsigned n = (unsigned)nx * ny * nz;
+    const int lane = threadIdx.x & 63;
+    for (unsigned base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < n;
+         base += gridDim.x * kBlock) {
+        const unsigned s = base + lane;
+        int k = 0;
+        bool wet = false;
+        if (s < n) wet = alpha[interior_index(s, nx, ny, fnx, fny, &k)] == 0;
+        unsigned long long pend = __ballot(wet);
+        while (pend) {
+            const int leader = __ffsll((long long)pend) - 1;
+            const int kl = __shfl(k, leader);
+            const unsigned long long same = __ballot(wet && k == kl) & pend;
+            if (lane == leader) liq[kl] = 1u;
+            pend &= ~same;
+        }
     }
 }
 
 // Line QoIs (QOI::calculateLineQoIs, 3DgrainMPI/QOI.cpp:71-90): per grain id
-// 1..ng, interior points (total) and those above the tip plane (extra).
+// 1..ng, interior points (total) and those above the tip plane (extra), with
+// one atomic per id present in a wave as in qoi_volume_kernel.
 __global__ __launch_bounds__(kBlock) void qoi_area_kernel(const int* __restrict__ alpha, int fnx,
                                                           int fny, int fnz, int ng, int tip,
                                                           int* total, int* extra)
 {
     const int nx = fnx - 2, ny = fny - 2, nz = fnz - 2;
-    const long long n = (long long)nx * ny * nz;
-    for (long long s = (long long)blockIdx.x * kBlock + threadIdx.x; s < n;
-         s += (long long)gridDim.x * kBlock) {
-        const int i = 1 + (int)(s % nx);
-        const long long r = s / nx;
-        const int j = 1 + (int)(r % ny);
-        const int k = 1 + (int)(r / ny);
-        const int a = alpha[(long long)k * fnx * fny + (long long)j * fnx + i];
-        if (a <= 0 || a > ng) continue;
-        atomicAdd(&total[a - 1], 1);
-        if (k > tip) atomicAdd(&extra[a - 1], 1);
+    const unsigned n = (unsigned)nx * ny * nz;
+    const int lane = threadIdx.x & 63;
+    for (unsigned base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < n;
+         base += gridDim.x * kBlock) {
+        const unsigned s = base + lane;
+        int a = 0, k = 0;
+        if (s < n) a = alpha[interior_index(s, nx, ny, fnx, fny, &k)];
+        const bool valid = a > 0 && a <= ng;
+        const unsigned long long up = __ballot(valid && k > tip);
+        unsigned long long pend = __ballot(valid);
+        while (pend) {
+            const int leader = __ffsll((long long)pend) - 1;
+            const int id = __shfl(a, leader);
+            const unsigned long long same = __ballot(valid && a == id) & pend;
+            if (lane == leader) {
+                atomicAdd(&total[id - 1], (int)__popcll(same));
+                const int e = (int)__popcll(same & up);
+                if (e) atomicAdd(&extra[id - 1], e);
+            }
+            pend &= ~same;
+        }
     }
 }
 
