@@ -6,18 +6,35 @@ field G=1e6 K/m, R=0.1 m/s, BC 110), state resident in HBM.  A step is one
 APTrhs_psi time step over all (nx+1)(ny+1)(nz+1) updated points.
 
 Prints one JSON line:
-  value      Mcell-steps/s over the timed steps (whole job, one GPU; every
-             updated point of the domain counts, as every one is updated)
-  roofline   the main step kernel's algorithmic HBM bytes over its average
-             HIP-event duration, against 8 TB/s.  The kernel skips z planes of
-             64x4 column tiles whose stencil reaches only empty slots and whose
-             destination is empty (their result is known without reading
-             anything), so its algorithmic bytes are those of the points it
-             processes: processed points x K x 16 B (one read and one write of
-             every slot's float32 phi and int32 id).  The same figure for all
-             updated points (the dense count) is given as dense_equiv_gbs.
-  cpu_baseline  the CPU oracle (float32 restatement, OpenMP) on the same
-             post-warmup state for a few steps
+  value      Mcell-steps/s over the timed steps (--warmup W untimed, then
+             --steps K timed; whole job, one GPU; every updated point of the
+             domain counts, as every one is updated)
+  roofline   per-kernel durations of the same window (the run is replayed
+             from the initial state with HIP events around every launch on
+             the context's stream): work-list plan, step kernel, overflow
+             fix-up.  The step kernel skips z planes of 64x4 column tiles
+             whose stencil reaches only empty slots and whose destination is
+             empty (their result is known without reading anything), so the
+             algorithmic bytes of a step are those of the points it processes:
+             processed points x K x 16 B (one read and one write of every
+             slot's float32 phi and int32 id), counted exactly per step from
+             the activity maps.  `kernel` names the kernel with the largest
+             share of the window; `achieved` divides the step's algorithmic
+             bytes by the summed per-launch duration of all three kernels
+             (the step's points are finished only when the fix-up is), and the
+             step kernel's own figure is given beside it.  `traffic` is the
+             measured HBM bytes per step of the same window (rocprof
+             FETCH_SIZE + WRITE_SIZE, profiles/traffic.json) or null.
+  dense      the same window with plane skipping off (every point read and
+             written): the step kernel's bytes (4.337 GB per step) over its
+             duration, the kernel-quality figure that skipping cannot flatter
+  full_run   the whole 10000-step simulation of this configuration (start-up
+             transient included): ms per step and Mcell-steps/s
+  cpu_baseline  the oracle (C restatement, OpenMP on all host threads) on
+             the state after the timed steps, float32 and float64 builds,
+             for a bounded sample of steps (time-capped); the GPU's next steps
+             from the same state are compared with both: arg mismatches and
+             L-inf(phi) against float32, arg flips and L-inf(phi) against float64
   ensemble   64 seeded simulations, grids drawn from 32^3, 64^3, 96x96x128 and
              128x128x256, G log-uniform in [1e5, 1e7] K/m, R log-uniform in
              [0.01, 1] m/s, K=8, 2000 steps each: batched (one launch sequence
@@ -194,18 +211,117 @@ def qoi_block(ctx, p) -> dict:
     }
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def new_ctx(case):
+    from cupf_amd import capi
+    ctx = capi.APTContext(case.params)
+    ctx.set_coords(case.x, case.y, case.z)
+    ctx.set_orientations(case.R9)
+    ctx.init_from_phi_alpha(case.phi0, case.alpha0)
+    return ctx
+
+
+def profiled_window(ctx, p, first: int, steps: int):
+    """Steps [first, first+steps) of ctx one at a time with HIP events around
+    every launch; processed points of each step from the activity maps just
+    before it.  Returns per-step lists (plan ms, step ms, fix-up ms, points)."""
+    plan, main, fix, pts = [], [], [], []
+    for q in range(steps):
+        pts.append(processed_points(ctx, p))
+        m, f, pl = ctx.step_profiled(1, first + q)
+        plan.append(pl)
+        main.append(m)
+        fix.append(f)
+    return plan, main, fix, pts
+
+
+def cpu_block(ctx, case, step0: int, budget_s: float, threads: int) -> dict:
+    """Oracle float32 and float64 from the GPU state after the timed steps, as
+    many steps as fit the time budget (at least one); then the GPU steps on
+    from the same state and is compared with both."""
+    import oracle
+    p = case.params
+    n_upd = (p["fnx"] - 2) * (p["fny"] - 2) * (p["fnz"] - 2)
+    ph, arg = ctx.download_state(0)
+    res = {}
+    nsteps = None
+    for prec in ("f32", "f64"):
+        sim = oracle.OracleSim(p, case.x, case.y, case.z, case.R9, prec)
+        sim.set_state(ph, arg, 0)
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            sim.step(1, step0 + n)
+            n += 1
+            el = time.perf_counter() - t0
+            if nsteps is not None:
+                if n >= nsteps:
+                    break
+            elif el >= budget_s / 2:
+                break
+        nsteps = n
+        sim.apply_bc(0)
+        res[prec] = (sim.state(0), n, el)
+        log(f"[bench] cpu oracle {prec}: {n} steps in {el:.1f} s, "
+            f"{n_upd * n / el / 1e6:.3f} Mcell-steps/s on {threads} threads")
+    ctx.step(nsteps, step0)
+    ctx.apply_bc(0)
+    gph, garg = ctx.download_state(0)
+    o32, a32 = res["f32"][0]
+    o64, a64 = res["f64"][0]
+    cmp = {
+        "gpu_vs_f32_arg_mismatches": int((garg != a32).sum()),
+        "gpu_vs_f32_linf_phi": float(np.abs(gph.astype(np.float64) - o32.astype(np.float64)).max()),
+        "gpu_vs_f64_arg_flips": int((garg != a64).sum()),
+        "gpu_vs_f64_linf_phi": float(np.abs(gph.astype(np.float64) - o64).max()),
+        "f32_vs_f64_arg_flips": int((a32 != a64).sum()),
+        "f32_vs_f64_linf_phi": float(np.abs(o32.astype(np.float64) - o64).max()),
+    }
+    log(f"[bench] accuracy after {nsteps} steps: {cmp}")
+    n32, e32 = res["f32"][1], res["f32"][2]
+    n64, e64 = res["f64"][1], res["f64"][2]
+    return {
+        "value": round(n_upd * n32 / e32 / 1e6, 3),
+        "unit": "Mcell-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "cpu_model": cpu_model(),
+        "f64_value": round(n_upd * n64 / e64 / 1e6, 3),
+        "sample": f"{nsteps} steps of the 256x256x512 K=8 state after the timed GPU steps "
+                  f"(steps {step0}..{step0 + nsteps - 1}), oracle float32 {e32:.1f} s and float64 "
+                  f"{e64:.1f} s, OpenMP {threads} threads",
+        "accuracy": cmp,
+        "accuracy_steps": nsteps,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=400)
     ap.add_argument("--K", type=int, default=8)
-    ap.add_argument("--cpu-steps", type=int, default=80)
+    ap.add_argument("--cpu-seconds", type=float, default=24.0,
+                    help="time budget of the CPU oracle sample (float32 and float64 together)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-only", type=int, default=0,
                     help="only time a plain copy kernel over this many bytes (counter calibration)")
     ap.add_argument("--z0-cells", type=float, default=None,
                     help="initial interface height in cells (default: z0 = 2 um as in line.py)")
+    ap.add_argument("--full-steps", type=int, default=10000,
+                    help="length of the full-run measurement (0: skip)")
+    ap.add_argument("--no-dense", action="store_true", help="skip the dense (no plane skipping) window")
+    ap.add_argument("--no-qoi", action="store_true", help="skip the device QoI timings")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--ensemble-sims", type=int, default=64)
@@ -236,11 +352,7 @@ def main() -> int:
     log(f"[bench] case {case.name}: fnx,fny,fnz={p['fnx']},{p['fny']},{p['fnz']} grains={case.n_grains} "
         f"K={args.K} updated points={n_upd} (setup {time.time() - t0:.1f}s)")
 
-    ctx = capi.APTContext(p)
-    ctx.set_coords(case.x, case.y, case.z)
-    ctx.set_orientations(case.R9)
-    ctx.init_from_phi_alpha(case.phi0, case.alpha0)
-
+    ctx = new_ctx(case)
     step0 = 1
     if args.warmup:
         ms = ctx.step_timed(args.warmup, step0)
@@ -254,55 +366,91 @@ def main() -> int:
     ev_ms = ctx.step_timed(args.steps, step0)
     ctx.sync()
     wall = time.perf_counter() - w0
-    step0 += args.steps
+    step_after = step0 + args.steps
     ovf = ctx.overflow_count() - ovf0
+    mcell = n_upd * args.steps / wall / 1e6
     log(f"[bench] timed {args.steps} steps: wall {wall * 1e3 / args.steps:.3f} ms/step, "
         f"events {ev_ms / args.steps:.3f} ms/step, overflow points {ovf}")
 
-    # per-kernel durations (HIP events around every launch on the context's stream)
-    nprof = min(args.steps, 100)
-    proc0 = processed_points(ctx, p)
-    ms_main, ms_fix, ms_plan = ctx.step_profiled(nprof, step0)
-    step0 += nprof
-    proc1 = processed_points(ctx, p)
-    proc = (proc0 + proc1) // 2
-    avg_main = ms_main / nprof
-    act, tot = ctx.activity(0)
-    log(f"[bench] per launch: apt_plan {ms_plan / nprof:.4f} ms, apt_step {avg_main:.4f} ms, "
-        f"fix-ups {ms_fix / nprof:.4f} ms; "
-        f"processed points {proc0}..{proc1} of {n_upd} ({proc / n_upd:.1%}), "
-        f"active tile planes {act}/{tot}")
+    # the same window replayed with events around every launch
+    rctx = new_ctx(case)
+    if args.warmup:
+        rctx.step(args.warmup, 1)
+    rovf0 = rctx.overflow_count()
+    plan, main_ms, fix, pts = profiled_window(rctx, p, step0, args.steps)
+    rovf = rctx.overflow_count() - rovf0
+    rctx.close()
+    nprof = len(main_ms)
+    avg_plan, avg_main, avg_fix = sum(plan) / nprof, sum(main_ms) / nprof, sum(fix) / nprof
+    proc = sum(pts) / nprof
+    log(f"[bench] per launch (same window): apt_plan {avg_plan:.4f} ms, apt_step {avg_main:.4f} ms, "
+        f"apt_fixup {avg_fix:.4f} ms; processed points {min(pts)}..{max(pts)} (mean {proc:.0f}) "
+        f"of {n_upd}; overflow points {rovf}")
 
     copy_gbs = capi.copy_bandwidth(1 << 30, 10)
     log(f"[bench] plain copy kernel: {copy_gbs:.0f} GB/s")
     bytes_step = proc * args.K * 16
     bytes_dense = n_upd * args.K * 16
-    achieved = bytes_step / (avg_main * 1e-3) / 1e9
-    dense_equiv = bytes_dense / (avg_main * 1e-3) / 1e9
-    mcell = n_upd * args.steps / wall / 1e6
+    t_all = avg_plan + avg_main + avg_fix
+    achieved = bytes_step / (t_all * 1e-3) / 1e9
+    achieved_main = bytes_step / (avg_main * 1e-3) / 1e9
+    dominant = "apt_step_tiled<%d,false>" % args.K if avg_main >= avg_fix else "apt_fixup<%d,false>" % args.K
+    traffic = traffic_from_profiles(f"line-256x256x512:w{args.warmup}:s{args.steps}")
+
+    dense = None
+    if not args.no_dense:
+        os.environ["CUPF_NO_ACTIVITY"] = "1"
+        dctx = new_ctx(case)
+        del os.environ["CUPF_NO_ACTIVITY"]
+        if args.warmup:
+            dctx.step(args.warmup, 1)
+        dm, df, dp = dctx.step_profiled(args.steps, step0)
+        g1 = ctx.download_state(0)
+        g2 = dctx.download_state(0)
+        same = bool(np.array_equal(g1[1], g2[1]) and np.array_equal(g1[0].view(np.uint32), g2[0].view(np.uint32)))
+        del g1, g2
+        dctx.close()
+        d_main = dm / args.steps
+        dense = {
+            "workload": "same window, every plane processed (CUPF_NO_ACTIVITY=1)",
+            "kernel_ms": round(d_main, 4),
+            "plan_ms": round(dp / args.steps, 4),
+            "fixup_ms": round(df / args.steps, 4),
+            "algorithmic_bytes_per_launch": bytes_dense,
+            "achieved_gbs": round(bytes_dense / (d_main * 1e-3) / 1e9, 1),
+            "frac": round(bytes_dense / (d_main * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "same_bits_as_skipping": same,
+        }
+        log(f"[bench] dense window: step kernel {d_main:.4f} ms ({dense['achieved_gbs']} GB/s), "
+            f"same bits as skipping: {same}")
+
+    full = None
+    if args.full_steps > 0:
+        fctx = new_ctx(case)
+        fctx.sync()
+        f0 = time.perf_counter()
+        fctx.step(args.full_steps, 1)
+        fctx.sync()
+        fsec = time.perf_counter() - f0
+        fovf = fctx.overflow_count()
+        fctx.close()
+        full = {
+            "workload": f"the whole {args.full_steps}-step run from the initial state",
+            "steps": args.full_steps,
+            "ms_per_step": round(fsec * 1e3 / args.full_steps, 4),
+            "mcell_steps_per_s": round(n_upd * args.full_steps / fsec / 1e6, 1),
+            "wall_s": round(fsec, 3),
+            "overflow_points": fovf,
+        }
+        log(f"[bench] full run: {args.full_steps} steps in {fsec:.2f} s "
+            f"({full['ms_per_step']} ms/step, {full['mcell_steps_per_s']} Mcell-steps/s)")
 
     cpu = None
-    if not args.no_cpu and args.cpu_steps > 0:
-        import oracle
-        ph, arg = ctx.download_state(0)
-        sim = oracle.OracleSim(p, case.x, case.y, case.z, case.R9, "f32")
-        sim.set_state(ph, arg, 0)
+    if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        c0 = time.perf_counter()
-        sim.step(args.cpu_steps, step0)
-        csec = time.perf_counter() - c0
-        cpu = {
-            "value": round(n_upd * args.cpu_steps / csec / 1e6, 3),
-            "unit": "Mcell-steps/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{args.cpu_steps} steps of the same 256x256x512 K=8 state after "
-                      f"{args.warmup} GPU warmup steps (float32 oracle, OpenMP {threads} threads, "
-                      f"{csec:.1f} s)",
-        }
-        log(f"[bench] cpu oracle: {cpu['value']} Mcell-steps/s on {threads} threads")
+        cpu = cpu_block(ctx, case, step_after, args.cpu_seconds, threads)
 
-    qoi = qoi_block(ctx, p)
+    qoi = None if args.no_qoi else qoi_block(ctx, p)
 
     ens = None
     if not args.no_ensemble:
@@ -334,6 +482,7 @@ def main() -> int:
             "R_m_per_s": 0.1,
             "bc": "110",
             "z0_um": float(p["z0"]),
+            "window": f"steps {step0}..{step_after - 1} after {args.warmup} warm-up steps",
             "overflow_points_timed": ovf,
         },
         "roofline": {
@@ -342,19 +491,24 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles("line-256x256x512"),
-            "kernel": "apt_step_tiled<8,false>",
+            "traffic": traffic,
+            "kernel": dominant,
+            "kernels": "apt_plan + apt_step_tiled + apt_fixup (one step)",
+            "step_ms": round(t_all, 4),
+            "plan_ms": round(avg_plan, 4),
+            "kernel_ms": round(avg_main, 4),
+            "fixup_ms": round(avg_fix, 4),
+            "step_kernel_achieved": round(achieved_main, 1),
+            "step_kernel_frac": round(achieved_main / HBM_PEAK_GBS, 4),
             "copy_kernel_gbs": round(copy_gbs, 1),
             "frac_of_copy": round(achieved / copy_gbs, 4),
-            "kernel_ms": round(avg_main, 4),
-            "plan_ms": round(ms_plan / nprof, 4),
-            "fixup_ms": round(ms_fix / nprof, 4),
-            "algorithmic_bytes_per_launch": bytes_step,
-            "processed_points_per_launch": proc,
+            "algorithmic_bytes_per_launch": int(bytes_step),
+            "processed_points_per_launch": int(proc),
             "updated_points_per_launch": n_upd,
-            "dense_algorithmic_bytes_per_launch": bytes_dense,
-            "dense_equiv_gbs": round(dense_equiv, 1),
+            "overflow_points_per_launch": int(rovf // max(1, args.steps)),
         },
+        "dense": dense,
+        "full_run": full,
         "cpu_baseline": cpu,
         "qoi": qoi,
         "ensemble": ens,

@@ -83,6 +83,9 @@ def load(precision: str = "f32") -> ctypes.CDLL:
         "orc_kine_ani": ([P, c_double, c_double, c_double, fp], c_double),
         "orc_thermal": ([P, fp, fp, fp, fp, c_int, c_int, c_int, c_double, c_int,
                          ctypes.POINTER(c_double)], c_int),
+        "orc_interp3": ([fp, c_double, c_double, c_double, c_int, c_int, c_int, c_double], c_double),
+        "orc_interp4": ([fp, c_double, c_double, c_double, c_double, c_int, c_int, c_int, c_int,
+                         c_double, c_double], c_double),
         "orc_plane_sums": ([P, vp, i64p], None),
         "orc_tip_scan": ([P, i64p, c_int], c_int),
         "orc_move_frame": ([P, vp, ip, vp, ip, fp, ip, ip, ip, c_int], None),

@@ -154,6 +154,18 @@ static REAL interp4(const float* u, REAL x, REAL y, REAL z, REAL t, int Nx, int 
     return (one - dt) * up + dt * un;
 }
 
+/* Exported for the oracle's own pins (exactness on multilinear fields). */
+double orc_interp3(const float* u, double x, double y, double z, int Nx, int Ny, int Nz, double Dx)
+{
+    return (double)interp3(u, (REAL)x, (REAL)y, (REAL)z, Nx, Ny, Nz, (REAL)Dx);
+}
+
+double orc_interp4(const float* u, double x, double y, double z, double t, int Nx, int Ny, int Nz,
+                   int Nt, double Dx, double Dt)
+{
+    return (double)interp4(u, (REAL)x, (REAL)y, (REAL)z, (REAL)t, Nx, Ny, Nz, Nt, (REAL)Dx, (REAL)Dt);
+}
+
 /* ------------------------------------------------------------- thermal --- */
 /* Thermal branch of APTrhs_psi (3DgrainMPI/APTPhaseField.cu:278-327) and the
  * 3Dgrain line law (3Dgrain/APTPhaseField.cu:303).  Literal double constants

@@ -72,6 +72,12 @@ int orc_thermal(const orc_params* P, const float* x, const float* y, const float
                 const float* psi_mac, int i, int j, int k, double t, int use_initial_uc,
                 double* T_out);
 
+/* interp3Dtemperature / interp4Dtemperature (3DgrainMPI/devicefunc.cu_inl:32-125),
+ * indices clamped into the macro grid. */
+double orc_interp3(const float* u, double x, double y, double z, int Nx, int Ny, int Nz, double Dx);
+double orc_interp4(const float* u, double x, double y, double z, double t, int Nx, int Ny, int Nz,
+                   int Nt, double Dx, double Dt);
+
 /* Per-plane sum of phi over all slots and (i,j) incl. halo as 2^-40 fixed point
  * (ave_x, 3DgrainMPI/PhaseField.cu:366-378, made order independent). */
 void orc_plane_sums(const orc_params* P, const void* ph, int64_t* sums);
