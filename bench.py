@@ -23,13 +23,15 @@ Prints one JSON line:
              bytes by the summed per-launch duration of all three kernels
              (the step's points are finished only when the fix-up is), and the
              step kernel's own figure is given beside it.  `traffic` is the
-             measured HBM bytes per step of the same window (rocprof
-             FETCH_SIZE + WRITE_SIZE, profiles/traffic.json) or null.
+             measured HBM bytes of one step of the same window (rocprof
+             FETCH_SIZE + WRITE_SIZE of plan, step and fix-up kernels,
+             profiles/traffic.json) or null if that window was not profiled.
   dense      the same window with plane skipping off (every point read and
              written): the step kernel's bytes (4.337 GB per step) over its
              duration, the kernel-quality figure that skipping cannot flatter
   full_run   the whole 10000-step simulation of this configuration (start-up
-             transient included): ms per step and Mcell-steps/s
+             transient included): ms per step and Mcell-steps/s, and per block
+             of steps (ms/step, processed points, overflowed points)
   cpu_baseline  the oracle (C restatement, OpenMP on all host threads) on
              the state after the timed steps, float32 and float64 builds,
              for a bounded sample of steps (time-capped); the GPU's next steps
@@ -66,15 +68,20 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(workload: str):
+def traffic_from_profiles(window: str):
+    """Measured HBM bytes of one step of this window (rocprof FETCH_SIZE and
+    WRITE_SIZE of the plan, step and fix-up kernels, scaled by the copy-kernel
+    calibration) and the per-kernel split, from profiles/traffic.json; None
+    for a window that was not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     try:
         with open(path) as f:
-            return json.load(f).get(workload, {}).get("hbm_bytes_per_launch")
-    except Exception:
-        return None
+            w = json.load(f).get(window, {})
+    except (OSError, ValueError):
+        return None, None
+    return w.get("hbm_bytes_per_step"), w.get("kernels")
 
 
 def _bc_map(i: int, n: int, bc: int) -> int:
@@ -395,7 +402,7 @@ def main() -> int:
     achieved = bytes_step / (t_all * 1e-3) / 1e9
     achieved_main = bytes_step / (avg_main * 1e-3) / 1e9
     dominant = "apt_step_tiled<%d,false>" % args.K if avg_main >= avg_fix else "apt_fixup<%d,false>" % args.K
-    traffic = traffic_from_profiles(f"line-256x256x512:w{args.warmup}:s{args.steps}")
+    traffic, traffic_kernels = traffic_from_profiles(f"line-256x256x512:w{args.warmup}:s{args.steps}")
 
     dense = None
     if not args.no_dense:
@@ -426,12 +433,29 @@ def main() -> int:
 
     full = None
     if args.full_steps > 0:
+        # the whole run in blocks (one stream sync between blocks): the
+        # start-up transient, the thin solid slab, then the slab growing as the
+        # front advances
         fctx = new_ctx(case)
-        fctx.sync()
-        f0 = time.perf_counter()
-        fctx.step(args.full_steps, 1)
-        fctx.sync()
-        fsec = time.perf_counter() - f0
+        edges = [b for b in (5, 25, 100, 200, 300, 500, 1000, 2000, 3000, 5000, 7000) if b < args.full_steps]
+        edges.append(args.full_steps)
+        blocks = []
+        fsec = 0.0
+        s = 1
+        for e in edges:
+            n = e - s + 1
+            pts = processed_points(fctx, p)
+            o0 = fctx.overflow_count()
+            fctx.sync()
+            f0 = time.perf_counter()
+            fctx.step(n, s)
+            fctx.sync()
+            dt = time.perf_counter() - f0
+            fsec += dt
+            blocks.append({"steps": [s, e], "ms_per_step": round(dt * 1e3 / n, 4),
+                           "processed_points_first_step": pts,
+                           "overflow_points_per_step": round((fctx.overflow_count() - o0) / n)})
+            s = e + 1
         fovf = fctx.overflow_count()
         fctx.close()
         full = {
@@ -441,9 +465,13 @@ def main() -> int:
             "mcell_steps_per_s": round(n_upd * args.full_steps / fsec / 1e6, 1),
             "wall_s": round(fsec, 3),
             "overflow_points": fovf,
+            "blocks": blocks,
         }
         log(f"[bench] full run: {args.full_steps} steps in {fsec:.2f} s "
             f"({full['ms_per_step']} ms/step, {full['mcell_steps_per_s']} Mcell-steps/s)")
+        for b in blocks:
+            log(f"[bench]   steps {b['steps'][0]}..{b['steps'][1]}: {b['ms_per_step']} ms/step, "
+                f"processed {b['processed_points_first_step']}, overflow/step {b['overflow_points_per_step']}")
 
     cpu = None
     if not args.no_cpu:
@@ -492,6 +520,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_by_kernel": traffic_kernels,
             "kernel": dominant,
             "kernels": "apt_plan + apt_step_tiled + apt_fixup (one step)",
             "step_ms": round(t_all, 4),

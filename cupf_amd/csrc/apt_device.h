@@ -17,8 +17,7 @@ namespace cupf {
 namespace {
 
 constexpr int kBlock = 256;        // 4 wave64 per workgroup
-constexpr int kFixupBlocks = 512;  // batched fix-up grid: 2 workgroups per CU
-constexpr int kFixupGridSingle = 1536;  // single-simulation fix-up grid: 6 workgroups per CU
+constexpr int kFixLanes = 256;     // fix-up workgroup
 
 // Halo index -> interior source index along one axis (APTsetBC3D,
 // 3DgrainMPI/APTPhaseField.cu:111-112,134-135,157-158): no-flux mirrors

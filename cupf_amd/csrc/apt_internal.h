@@ -74,6 +74,7 @@ struct StepArgs {
     int edge_cols;         // interior columns outside the full tiles
     int edge_items;        // edge items per z chunk: ceil(kchunk * edge_cols / kEdgePoints)
     int step_blocks;       // persistent workgroups of the step kernel
+    int fix_blocks;        // workgroups (one wave each) of the overflow fix-up kernel
     int fnx, fny, fnz;
     int L;                 // slot stride fnx*fny*fnz
     int ncolx, ncols;      // interior columns: (fnx-2), (fnx-2)*(fny-2)
@@ -150,6 +151,7 @@ struct BatchArgs {
     unsigned* cnt;
     int wl_cap;            // list capacity: light chunks are listed down from wl_cap - 1
     int step_blocks;
+    int fix_blocks;        // workgroups (one wave each) of the batched fix-up kernel
 };
 
 // Nucleation launch block (3DgrainMPI/APTPhaseField.cu:459-536).
@@ -182,6 +184,8 @@ struct NuclArgs {
 // Workgroups of the persistent step kernel that fit the device at once.
 int step_kernel_slots(int K, int gather27, int device);
 int batch_kernel_slots(int K, int gather27, int device);
+// Workgroups of the (single or batched) overflow fix-up kernel that fit at once.
+int fixup_kernel_slots(int K, int gather27, int device, bool batch);
 // Batched step: plan, persistent step, fix-up and advance kernels.
 hipError_t launch_batch_step(const BatchArgs& b, int K, int gather27, hipStream_t stream);
 // Pieces shared by the per-K launchers.
@@ -191,19 +195,19 @@ hipError_t launch_batch_advance(const BatchArgs& b, hipStream_t stream);
 hipError_t launch_step_k5_g7(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
                              hipEvent_t e2, hipEvent_t e3);
 hipError_t launch_batch_k5_g7(const BatchArgs& b, hipStream_t stream);
-int step_slots_k5_g7(int device, bool batch);
+int step_slots_k5_g7(int device, int which);
 hipError_t launch_step_k5_g27(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
                              hipEvent_t e2, hipEvent_t e3);
 hipError_t launch_batch_k5_g27(const BatchArgs& b, hipStream_t stream);
-int step_slots_k5_g27(int device, bool batch);
+int step_slots_k5_g27(int device, int which);
 hipError_t launch_step_k8_g7(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
                              hipEvent_t e2, hipEvent_t e3);
 hipError_t launch_batch_k8_g7(const BatchArgs& b, hipStream_t stream);
-int step_slots_k8_g7(int device, bool batch);
+int step_slots_k8_g7(int device, int which);
 hipError_t launch_step_k8_g27(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
                              hipEvent_t e2, hipEvent_t e3);
 hipError_t launch_batch_k8_g27(const BatchArgs& b, hipStream_t stream);
-int step_slots_k8_g27(int device, bool batch);
+int step_slots_k8_g27(int device, int which);
 // Work-list plan, main step kernel, then the overflow fix-up kernels.
 // Optional events are recorded before the plan, after it, after the main
 // kernel and after the fix-ups.

@@ -497,16 +497,21 @@ hipError_t launch_batch_step(const BatchArgs& b, int K, int gather27, hipStream_
     return hipErrorInvalidValue;
 }
 
-static int slots(int K, int gather27, int device, bool batch)
+static int slots(int K, int gather27, int device, int which)
 {
-    if (K == 5) return gather27 ? step_slots_k5_g27(device, batch) : step_slots_k5_g7(device, batch);
-    if (K == 8) return gather27 ? step_slots_k8_g27(device, batch) : step_slots_k8_g7(device, batch);
+    if (K == 5) return gather27 ? step_slots_k5_g27(device, which) : step_slots_k5_g7(device, which);
+    if (K == 8) return gather27 ? step_slots_k8_g27(device, which) : step_slots_k8_g7(device, which);
     return 0;
 }
 
-int step_kernel_slots(int K, int gather27, int device) { return slots(K, gather27, device, false); }
+int step_kernel_slots(int K, int gather27, int device) { return slots(K, gather27, device, 0); }
 
-int batch_kernel_slots(int K, int gather27, int device) { return slots(K, gather27, device, true); }
+int batch_kernel_slots(int K, int gather27, int device) { return slots(K, gather27, device, 1); }
+
+int fixup_kernel_slots(int K, int gather27, int device, bool batch)
+{
+    return slots(K, gather27, device, batch ? 3 : 2);
+}
 
 hipError_t launch_act_rebuild(const float* ph, const int* arg, unsigned* act, unsigned ep, int fnx,
                               int fny, int fnz, int K, int ntx, int ntiles, hipStream_t stream)

@@ -66,6 +66,7 @@ struct pf_ctx {
     int nzchunk = 0, edge_cols = 0, edge_items = 0, items_chunk = 0;
     int tiles_x = 0, tiles_y = 0;   // LDS tiles (partial ones included)
     int step_blocks = 0;            // persistent step workgroups (occupancy x CUs)
+    int fix_blocks = 0;             // fix-up workgroups (occupancy x CUs)
     // overflow on the LDS path by tile plane (see StepArgs)
     bool have_coords = false, have_orient = false, have_state = false;
 };
@@ -157,6 +158,7 @@ StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
     a.edge_items = c->edge_items;
     a.items_chunk = c->items_chunk;
     a.step_blocks = c->step_blocks;
+    a.fix_blocks = c->fix_blocks;
     a.wl_item = c->wl_item;
     a.wl_need = c->wl_need;
     a.wl_act = c->wl_act;
@@ -300,6 +302,7 @@ struct pf_batch {
     hipStream_t stream = nullptr;
     int kchunk = 4;
     int step_blocks = 0;
+    int fix_blocks = 0;
     BatchSim* sims = nullptr;       // device array, n entries
     int* wl_sim = nullptr;
     int* wl_item = nullptr;
@@ -346,6 +349,7 @@ int batch_upload(pf_batch* b, const std::vector<int>& first_step, int use_iuc, B
         m.a.nzchunk = (p.fnz - 2 + b->kchunk - 1) / b->kchunk;
         m.a.edge_items = (int)(((long long)b->kchunk * c->edge_cols + kEdgePoints - 1) / kEdgePoints);
         m.a.step_blocks = b->step_blocks;
+        m.a.fix_blocks = b->fix_blocks;
         for (int q = 0; q < 2; ++q) {
             m.ph[q] = c->ph[q];
             m.arg[q] = c->arg[q];
@@ -376,6 +380,7 @@ int batch_upload(pf_batch* b, const std::vector<int>& first_step, int use_iuc, B
     ba.wl_cap = (int)b->wl_cap;
     ba.cnt = b->cnt;
     ba.step_blocks = b->step_blocks;
+    ba.fix_blocks = b->fix_blocks;
     return PF_OK;
 }
 
@@ -497,9 +502,16 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         c->nzchunk = (p.fnz - 2 + c->kchunk - 1) / c->kchunk;
     }
     c->step_blocks = step_kernel_slots(p.K, p.gather27, device);
-    if (c->step_blocks < 1) {
+    c->fix_blocks = fixup_kernel_slots(p.K, p.gather27, device, false);
+    if (c->step_blocks < 1 || c->fix_blocks < 1) {
         delete c;
-        return fail(PF_E_HIP, "occupancy query for the step kernel failed");
+        return fail(PF_E_HIP, "occupancy query for the step or fix-up kernel failed");
+    }
+    // fewer fix-up workgroups (each lane then redoes more overflowed points in
+    // turn); the results do not depend on it
+    if (const char* fb = getenv("CUPF_FIX_BLOCKS")) {
+        const int v = atoi(fb);
+        if (v >= 1 && v < c->fix_blocks) c->fix_blocks = v;
     }
     c->device = device;
     c->L = (int)L;
@@ -1157,11 +1169,23 @@ int pf_batch_create(pf_ctx* const* ctxs, int n, pf_batch** out)
         if (v >= pf_batch::kMinChunk && v <= 61) b->kchunk = v;
     }
     b->step_blocks = batch_kernel_slots(b->K, b->g27, b->device);
+    // 2 fix-up workgroups per CU (3.60 s for the 64-simulation ensemble, against 3.63-3.65 s
+    // with the 4 or 6 per CU that fit)
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess)
+            b->fix_blocks = std::min(fixup_kernel_slots(b->K, b->g27, b->device, true), 2 * ncu);
+    }
     auto bail = [&](int code) {
         batch_free(b);
         return code;
     };
-    if (b->step_blocks < 1) return bail(fail(PF_E_HIP, "occupancy query for the batch kernel failed"));
+    if (b->step_blocks < 1 || b->fix_blocks < 1)
+        return bail(fail(PF_E_HIP, "occupancy query for the batch step or fix-up kernel failed"));
+    if (const char* fb = getenv("CUPF_FIX_BLOCKS")) {
+        const int v = atoi(fb);
+        if (v >= 1 && v < b->fix_blocks) b->fix_blocks = v;
+    }
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(PF_E_HIP, "hipStreamCreate failed"));
     for (pf_ctx* c : b->ctxs)

@@ -14,6 +14,6 @@ hipError_t launch_batch_k8_g27(const BatchArgs& b, hipStream_t stream)
     return launch_batch_t<8, true>(b, stream);
 }
 
-int step_slots_k8_g27(int device, bool batch) { return slots_t<8, true>(device, batch); }
+int step_slots_k8_g27(int device, int which) { return slots_t<8, true>(device, which); }
 
 }  // namespace cupf

@@ -63,16 +63,16 @@ __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
 // lives in this step's counter set, which the next-but-one plan kernel
 // clears, so nothing here has to wait for the other workgroups.
 template <int K, bool G27>
-__global__ __launch_bounds__(kBlock) void apt_fixup(StepArgs a_arg)
+__global__ __launch_bounds__(kFixLanes) void apt_fixup(StepArgs a_arg)
 {
     // The arguments are read from an LDS copy: with them in scalar registers
     // the compiler keeps 229 VGPRs live (2 waves/SIMD); from LDS it needs 73
     // (6 waves/SIMD), and each entry is a long chain of dependent loads that
-    // only more waves in flight can overlap (with kFixupGridSingle workgroups
-    // the start-up transient of the 256x256x512 run is 14 % faster).
+    // only more waves in flight can overlap (6 workgroups per CU make the
+    // start-up transient of the 256x256x512 run 14 % faster than 2).
     __shared__ StepArgs s_a;
     constexpr int kWords = (int)(sizeof(StepArgs) / 4);
-    static_assert(sizeof(StepArgs) % 4 == 0 && kWords <= kBlock, "StepArgs staging");
+    static_assert(sizeof(StepArgs) % 4 == 0 && kWords <= kFixLanes, "StepArgs staging");
     if ((int)threadIdx.x < kWords)
         reinterpret_cast<unsigned*>(&s_a)[threadIdx.x] =
             reinterpret_cast<const unsigned*>(&a_arg)[threadIdx.x];
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kBlock) void apt_fixup(StepArgs a_arg)
     const float epsv = -4.0f * a.kin_delta / a_s;
     const int A = a.fnx * a.fny;
 #pragma unroll 1
-    for (unsigned q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+    for (unsigned q = blockIdx.x * kFixLanes + threadIdx.x; q < n; q += gridDim.x * kFixLanes) {
         const int C = (int)a.fix_list[q];
         const int k = C / A;
         const int r = C - k * A;
@@ -166,10 +166,10 @@ __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
 // (prefix sums of their lengths, in LDS) that the grid strides over, so long
 // and short lists alike are spread over every workgroup.
 template <int K, bool G27>
-__global__ __launch_bounds__(kBlock) void batch_fixup(BatchArgs b)
+__global__ __launch_bounds__(kFixLanes) void batch_fixup(BatchArgs b)
 {
     __shared__ unsigned s_pre[kMaxBatch + 1];
-    for (int t = threadIdx.x; t < b.n; t += kBlock)
+    for (int t = threadIdx.x; t < b.n; t += kFixLanes)
         s_pre[t + 1] = __atomic_load_n(b.sims[t].a.fix_count, __ATOMIC_RELAXED);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kBlock) void batch_fixup(BatchArgs b)
     }
     __syncthreads();
     const unsigned total = s_pre[b.n];
-    for (unsigned g = blockIdx.x * kBlock + threadIdx.x; g < total; g += gridDim.x * kBlock) {
+    for (unsigned g = blockIdx.x * kFixLanes + threadIdx.x; g < total; g += gridDim.x * kFixLanes) {
         int lo = 0, hi = b.n - 1;  // last simulation whose range starts at or before g
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -206,14 +206,14 @@ template <int K, bool G27>
 hipError_t launch_step_t(const StepArgs& a, hipStream_t stream, hipEvent_t ev_begin,
                          hipEvent_t ev_planned, hipEvent_t ev_mid, hipEvent_t ev_end)
 {
-    dim3 grid(a.step_blocks), block(kBlock), fgrid(kFixupGridSingle);
+    dim3 grid(a.step_blocks), block(kBlock), fgrid(a.fix_blocks), fblock(kFixLanes);
     if (ev_begin) (void)hipEventRecord(ev_begin, stream);
     hipError_t e = launch_plan(a, stream);
     if (e != hipSuccess) return e;
     if (ev_planned) (void)hipEventRecord(ev_planned, stream);
     hipLaunchKernelGGL((apt_step_tiled<K, G27>), grid, block, 0, stream, a);
     if (ev_mid) (void)hipEventRecord(ev_mid, stream);
-    hipLaunchKernelGGL((apt_fixup<K, G27>), fgrid, block, 0, stream, a);
+    hipLaunchKernelGGL((apt_fixup<K, G27>), fgrid, fblock, 0, stream, a);
     if (ev_end) (void)hipEventRecord(ev_end, stream);
     return hipGetLastError();
 }
@@ -225,21 +225,27 @@ hipError_t launch_batch_t(const BatchArgs& b, hipStream_t stream)
     hipError_t e = launch_batch_plan(b, stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((batch_step<K, G27>), grid, block, 0, stream, b);
-    hipLaunchKernelGGL((batch_fixup<K, G27>), dim3(kFixupBlocks), block, 0, stream, b);
+    hipLaunchKernelGGL((batch_fixup<K, G27>), dim3(b.fix_blocks), dim3(kFixLanes), 0, stream, b);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_batch_advance(b, stream);
 }
 
+// Workgroups that fit the device at once: which = 0 step, 1 batched step,
+// 2 fix-up, 3 batched fix-up.
 template <int K, bool G27>
-int slots_t(int device, bool batch)
+int slots_t(int device, int which)
 {
     int ncu = 0, per = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return 0;
-    const hipError_t e =
-        batch ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_step<K, G27>, kBlock, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<K, G27>, kBlock, 0);
+    hipError_t e;
+    switch (which) {
+    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<K, G27>, kBlock, 0); break;
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_step<K, G27>, kBlock, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_fixup<K, G27>, kFixLanes, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_fixup<K, G27>, kFixLanes, 0); break;
+    }
     if (e != hipSuccess || per < 1) return 0;
     return per * ncu;
 }

@@ -1,0 +1,74 @@
+"""Overflow fix-up (neighbourhoods holding more than K distinct grain ids,
+3DgrainMPI/APTPhaseField.cu:222-231) through the per-lane LDS tables.
+
+The cases have the 256x256x512 benchmark's grain density (grains ~3.6 cells
+wide, so most points of the start-up transient overflow), grain ids spread over
+[1, 2^31 - 2] (the range nucleated ids take), and the fix-up grid cut to two
+workgroups, so that every lane takes dozens of overflowed points in turn and
+reuses its table for each.  Bar: ids and phi bit-exact against the oracle and
+equal overflow counts; batched members equal to their single runs.
+"""
+import numpy as np
+import pytest
+
+from cupf_amd import capi, cases, ensemble
+from tests.helpers import compare_states, first_mismatch, gpu_context, oracle_sim
+
+pytestmark = pytest.mark.gpu
+
+
+def _dense_case(name, n, seed, K=8):
+    grains = int(round(n[0] * n[1] / 3.6 ** 2))
+    c = cases.make_line_case(name, n=n, grains=grains, K=K, seed=seed)
+    rng = np.random.default_rng(seed)
+    ids = np.unique(c.alpha0[c.alpha0 > 0])
+    while True:
+        new = rng.integers(1, 2 ** 31 - 1, size=ids.size, dtype=np.int64)
+        if np.unique(new).size == new.size:
+            break
+    lut = np.zeros(int(ids.max()) + 1, np.int64)
+    lut[ids] = new
+    a = c.alpha0.astype(np.int64)
+    a[a > 0] = lut[a[a > 0]]
+    c.alpha0[:] = a.astype(np.int32)
+    return c
+
+
+@pytest.mark.parametrize("K", [5, 8])
+def test_fixup_tables_reused_with_large_ids(K, monkeypatch):
+    monkeypatch.setenv("CUPF_FIX_BLOCKS", "2")
+    c = _dense_case("ovf-large-ids", (48, 48, 24), seed=17, K=K)
+    assert int(c.alpha0.max()) > 1 << 24
+    ctx = gpu_context(c)
+    sim = oracle_sim(c)
+    s = 1
+    for stop in (5, 12, 20):
+        ctx.step(stop - s + 1, s)
+        sim.step(stop - s + 1, s)
+        s = stop + 1
+        print(f"[ovf K={K}] step {stop}: overflow {sim.overflow}", flush=True)
+        assert ctx.overflow_count() == sim.overflow
+        bad, dphi = compare_states(ctx, sim)
+        p = c.params
+        assert bad == 0, first_mismatch(ctx, sim, K, p["fnx"], p["fny"], p["fnz"])
+        assert dphi == 0.0
+    # far more overflowed points than the two workgroups' 128 lanes
+    assert sim.overflow > 20 * 128
+
+
+def test_batched_fixup_tables_with_large_ids(monkeypatch):
+    monkeypatch.setenv("CUPF_FIX_BLOCKS", "2")
+    members = [_dense_case("ovf-b0", (40, 40, 20), seed=5), _dense_case("ovf-b1", (24, 56, 16), seed=6)]
+    alone = []
+    for c in members:
+        ctx = gpu_context(c)
+        ctx.step(15, 1)
+        alone.append(ensemble.state_hash(ctx))
+        assert ctx.overflow_count() > 0
+        ctx.close()
+    ctxs = [gpu_context(c) for c in members]
+    b = capi.Batch(ctxs)
+    b.step(15)
+    b.close()
+    for q, ctx in enumerate(ctxs):
+        assert ensemble.state_hash(ctx) == alone[q], f"member {q} differs from its single run"
