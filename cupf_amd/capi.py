@@ -105,6 +105,7 @@ def load() -> ctypes.CDLL:
         "pf_step_profiled": ([vp, c_int, c_int, c_int, fp, fp, fp], c_int),
         "pf_collect": ([vp, c_int, c_int, fp, ip], c_int),
         "pf_qoi_volume": ([vp, c_int, ip, ip, llp], c_int),
+        "pf_qoi_volume_noalloc": ([vp, c_int, ip, ip, llp], c_int),
         "pf_qoi_junctions3d": ([vp, ip, c_int, ip], c_int),
         "pf_qoi_line": ([vp, c_int, ip, ip, ip], c_int),
         "pf_md_enable": ([vp, c_int], c_int),
@@ -268,6 +269,17 @@ class APTContext:
         first = np.zeros(max_id + 1, dtype=np.int32)
         above = ctypes.c_longlong()
         _check(load().pf_qoi_volume(self._h, max_id, _iptr(counts), _iptr(first), ctypes.byref(above)))
+        ids = np.nonzero(counts)[0]
+        ids = ids[np.argsort(first[ids], kind="stable")]
+        return ids.tolist(), counts[ids].tolist(), above.value
+
+    def qoi_volume_noalloc(self, max_id: int):
+        """qoi_volume through pf_qoi_volume_noalloc (no device allocation;
+        max_id <= PF_NOALLOC_MAX_ID)."""
+        counts = np.zeros(max_id + 1, dtype=np.int32)
+        first = np.zeros(max_id + 1, dtype=np.int32)
+        above = ctypes.c_longlong()
+        _check(load().pf_qoi_volume_noalloc(self._h, max_id, _iptr(counts), _iptr(first), ctypes.byref(above)))
         ids = np.nonzero(counts)[0]
         ids = ids[np.argsort(first[ids], kind="stable")]
         return ids.tolist(), counts[ids].tolist(), above.value

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <array>
 #include <tuple>
 #include <cmath>
@@ -100,6 +101,39 @@ int dalloc(T** p, size_t n)
                                     " bytes failed: " + hipGetErrorString(e));
     }
     return PF_OK;
+}
+
+// Device memory of pf_qoi_volume_noalloc: counts[PF_NOALLOC_MAX_ID + 1],
+// first[PF_NOALLOC_MAX_ID + 1], then the 64-bit count of larger ids.  Host code
+// never uses the name `scratch` as an address (it is a host-side stand-in):
+// pf_create resolves the device address once per GPU with
+// hipGetSymbolAddress and keeps it in g_scratch.
+constexpr size_t kScratchIds = (size_t)PF_NOALLOC_MAX_ID + 1;
+static __device__ __align__(256) unsigned char scratch[2 * kScratchIds * sizeof(int) + 256];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxScratchDevices] = {};
+
+// Resolve this GPU's copy of `scratch` (the device must be current).
+int scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxScratchDevices)
+        return fail(PF_E_INVALID, "device index beyond the scratch table");
+    if (g_scratch[device]) return PF_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(PF_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = ptr;
+    return PF_OK;
+}
+
+void* scratch_of(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return (device >= 0 && device < kMaxScratchDevices) ? g_scratch[device] : nullptr;
 }
 
 int ctx_check(pf_ctx* c)
@@ -473,6 +507,7 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         return fail(PF_E_HIP, "no HIP device available");
     if (device < 0 || device >= ndev) return fail(PF_E_INVALID, "device index out of range");
     HIPCHK(hipSetDevice(device));
+    if (int rs = scratch_setup(device)) return rs;
 
     pf_ctx* c = new pf_ctx();
     c->p = p;
@@ -859,6 +894,35 @@ int pf_qoi_volume(pf_ctx* c, int max_id, int* counts, int* first, long long* n_a
     (void)hipFree(df);
     (void)hipFree(da);
     if (e != hipSuccess) return fail(PF_E_HIP, std::string("qoi volume: ") + hipGetErrorString(e));
+    if (n_above) *n_above = (long long)above;
+    return PF_OK;
+}
+
+int pf_qoi_volume_noalloc(pf_ctx* c, int max_id, int* counts, int* first, long long* n_above)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (max_id < 1 || !counts || !first) return fail(PF_E_INVALID, "max_id >= 1, counts and first required");
+    if (max_id > PF_NOALLOC_MAX_ID)
+        return fail(PF_E_INVALID, "max_id " + std::to_string(max_id) + " exceeds PF_NOALLOC_MAX_ID (" +
+                                      std::to_string(PF_NOALLOC_MAX_ID) + ") of pf_qoi_volume_noalloc");
+    void* base = scratch_of(c->device);
+    if (!base) return fail(PF_E_HIP, "no scratch address for this device");
+    const pf_params& p = c->p;
+    int* dc = static_cast<int*>(base);
+    int* df = dc + kScratchIds;
+    auto* da = reinterpret_cast<unsigned long long*>(df + kScratchIds);
+    // the launcher's fill kernels clear counts[0..max_id], first[0..max_id]
+    // and the count of larger ids before the count kernel runs
+    unsigned long long above = 0;
+    hipError_t e = launch_qoi_volume(c->alpha, p.fnx, p.fny, p.fnz, max_id, dc, df, da, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(counts, dc, sizeof(int) * ((size_t)max_id + 1), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(first, df, sizeof(int) * ((size_t)max_id + 1), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&above, da, sizeof(above), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(PF_E_HIP, std::string("qoi volume (no alloc): ") + hipGetErrorString(e));
     if (n_above) *n_above = (long long)above;
     return PF_OK;
 }

@@ -160,6 +160,18 @@ int pf_collect(pf_ctx* ctx, int which, int flags, float* phi_out, int* alpha_out
  * *n_above = interior points whose alpha exceeds max_id (nucleated ids). */
 int pf_qoi_volume(pf_ctx* ctx, int max_id, int* counts, int* first, long long* n_above);
 
+/* No counterpart in the reference.  pf_qoi_volume for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same results, with the per-grain
+ * buffers in a fixed device array of this library instead of allocations
+ * (PF_NOALLOC_MAX_ID + 1 counts and first indices plus one 64-bit count:
+ * 8 MiB per GPU, held until the process ends; its address is resolved once
+ * per GPU by pf_create).  max_id above PF_NOALLOC_MAX_ID is rejected with
+ * PF_E_INVALID.  Every call on one GPU uses that GPU's array, so calls that
+ * overlap on one GPU (from several threads or streams) race. */
+#define PF_NOALLOC_MAX_ID 1048575
+int pf_qoi_volume_noalloc(pf_ctx* ctx, int max_id, int* counts, int* first, long long* n_above);
+
 /* QOI3D::searchJunctionsOnImage (3DgrainMPI/QOI.cpp:233-311): solid interior
  * points whose 19-point neighbourhood holds no liquid and exactly 4 distinct
  * ids, none more than 10 times.  Up to cap rows of 8 ints (i, j, k, largest
