@@ -108,6 +108,7 @@ def load() -> ctypes.CDLL:
         "pf_qoi_volume_noalloc": ([vp, c_int, ip, ip, llp], c_int),
         "pf_qoi_junctions3d": ([vp, ip, c_int, ip], c_int),
         "pf_qoi_line": ([vp, c_int, ip, ip, ip], c_int),
+        "pf_qoi_tip": ([vp, c_int, ip], c_int),
         "pf_md_enable": ([vp, c_int], c_int),
         "pf_md_tip": ([vp, ip], c_int),
         "pf_md_shift": ([vp, c_int], c_int),
@@ -283,6 +284,12 @@ class APTContext:
         ids = np.nonzero(counts)[0]
         ids = ids[np.argsort(first[ids], kind="stable")]
         return ids.tolist(), counts[ids].tolist(), above.value
+
+    def qoi_tip(self, cur_tip: int) -> int:
+        """Device QOI::sampleHeights on the collected alpha: the tip plane after cur_tip."""
+        tip = ctypes.c_int()
+        _check(load().pf_qoi_tip(self._h, cur_tip, ctypes.byref(tip)))
+        return tip.value
 
     def qoi_line(self, cur_tip: int):
         """Device line QoIs on the collected alpha: (new tip plane, total and

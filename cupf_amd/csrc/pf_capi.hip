@@ -962,6 +962,48 @@ int pf_qoi_junctions3d(pf_ctx* c, int* rows, int cap, int* count)
     return PF_OK;
 }
 
+namespace {
+
+// The tip plane after cur_tip on the collected alpha (3DgrainMPI/QOI.cpp:47-67,
+// 162-185): per-plane liquid flags on the device, the scan over fnz flags on
+// the host.  Leaves the flags' allocation to the caller's dl (fnz words).
+hipError_t device_tip(pf_ctx* c, unsigned* dl, int cur_tip, int* tip)
+{
+    const pf_params& p = c->p;
+    std::vector<unsigned> liq((size_t)p.fnz, 0u);
+    hipError_t e = launch_qoi_liquid_planes(c->alpha, p.fnx, p.fny, p.fnz, dl, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(liq.data(), dl, sizeof(unsigned) * p.fnz, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    int t = cur_tip;
+    for (;;) {  // step up to the first plane holding liquid
+        ++t;
+        if (t >= p.fnz - 1 || liq[t]) break;
+    }
+    *tip = t - 1;
+    return hipSuccess;
+}
+
+}  // namespace
+
+int pf_qoi_tip(pf_ctx* c, int cur_tip, int* new_tip)
+{
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    const pf_params& p = c->p;
+    if (!new_tip) return fail(PF_E_INVALID, "new_tip required");
+    if (cur_tip < 0 || cur_tip > p.fnz - 2) return fail(PF_E_INVALID, "cur_tip out of range");
+    unsigned* dl = nullptr;
+    if ((rc = dalloc(&dl, (size_t)p.fnz))) return rc;
+    int tip = cur_tip;
+    const hipError_t e = device_tip(c, dl, cur_tip, &tip);
+    (void)hipFree(dl);
+    if (e != hipSuccess) return fail(PF_E_HIP, std::string("qoi tip: ") + hipGetErrorString(e));
+    *new_tip = tip;
+    return PF_OK;
+}
+
 int pf_qoi_line(pf_ctx* c, int cur_tip, int* new_tip, int* total, int* extra)
 {
     int rc = ctx_check(c);
@@ -978,21 +1020,9 @@ int pf_qoi_line(pf_ctx* c, int cur_tip, int* new_tip, int* total, int* extra)
         (void)hipFree(dt);
         return rc;
     }
-    std::vector<unsigned> liq((size_t)p.fnz, 0u);
-    hipError_t e = launch_qoi_liquid_planes(c->alpha, p.fnx, p.fny, p.fnz, dl, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(liq.data(), dl, sizeof(unsigned) * p.fnz, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     int tip = cur_tip;
-    if (e == hipSuccess) {
-        // 3DgrainMPI/QOI.cpp:47-67: step up to the first plane holding liquid
-        for (;;) {
-            ++tip;
-            if (tip >= p.fnz - 1 || liq[tip]) break;
-        }
-        tip -= 1;
-        e = launch_qoi_area(c->alpha, p.fnx, p.fny, p.fnz, ng, tip, dt, de, c->stream);
-    }
+    hipError_t e = device_tip(c, dl, cur_tip, &tip);
+    if (e == hipSuccess) e = launch_qoi_area(c->alpha, p.fnx, p.fny, p.fnz, ng, tip, dt, de, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(total, dt, sizeof(int) * ng, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess)

@@ -32,14 +32,20 @@ def _txt(path, arr, fmt="%1.4e"):
 
 def write_line_case(root: str, seed: int, *, n: tuple, grains: int, G: float, R: float,
                     z0: float, t_end: float, nts: int = 2, line_config: bool = False,
-                    nz_full: int | None = None, top: float = 48.0, macro=(13, 13, 23, 11)) -> dict:
+                    nz_full: int | None = None, top: float = 48.0, macro=(13, 13, 23, 11),
+                    inp_overrides: dict | None = None, nmax: float | None = None,
+                    temp=0.0) -> dict:
     """Write <root>/line.py and <root>/forcing/case<seed>/.  G in K/um, R in m/s,
-    z0/top in um, t_end in s.  Returns the input dict."""
+    z0/top in um, t_end in s.  inp_overrides replace keys of the input file
+    (after the grid keys), nmax is Nmax.txt (default: the input's nuc_Nmax) and
+    temp the macro temperature Temp.h5:/Temp, uniform in space: one value, or
+    one value per time frame of t.txt.  Returns the input dict."""
     nx, ny, nz = n
     inp = dict(cases.LINE_PY)
     Lx = nx * 0.8 * 0.1
     inp.update(Lx=round(Lx, 6), asp_ratio_yx=ny / nx, asp_ratio_zx=(nz_full or nz) / nx,
                moving_ratio=nz / nx, nts=nts, nx=macro[0], ny=macro[1], nz=macro[2], nt=macro[3])
+    inp.update(inp_overrides or {})
     os.makedirs(root, exist_ok=True)
     write_input_py(os.path.join(root, "line.py"), inp)
     folder = os.path.join(root, "forcing", f"case{seed}")
@@ -58,13 +64,15 @@ def write_line_case(root: str, seed: int, *, n: tuple, grains: int, G: float, R:
     ng = int(img.max())
     theta = cases.reference_angles(ng, seed)
     Rmax_um = R * 1e6
-    T = np.zeros(Nx * Ny * Nz * Nt, np.float32)
+    temps = np.broadcast_to(np.asarray(temp, np.float32), (Nt,))
+    T = np.repeat(temps, Nx * Ny * Nz).astype(np.float32)  # time is the slowest index
     _txt(f"{folder}/x.txt", x); _txt(f"{folder}/y.txt", y); _txt(f"{folder}/z.txt", z)
     _txt(f"{folder}/t.txt", t); _txt(f"{folder}/psi.txt", psi)
     _txt(f"{folder}/alpha.txt", img.ravel(), "%d")
     _txt(f"{folder}/theta.txt", theta)
     _txt(f"{folder}/G.txt", [G]); _txt(f"{folder}/Rmax.txt", [Rmax_um * 1e-6]); _txt(f"{folder}/V.txt", [0.0])
-    _txt(f"{folder}/Nmax.txt", [inp["nuc_Nmax"]]); _txt(f"{folder}/UC.txt", [G * Rmax_um / 1e6])
+    _txt(f"{folder}/Nmax.txt", [inp["nuc_Nmax"] if nmax is None else nmax])
+    _txt(f"{folder}/UC.txt", [G * Rmax_um / 1e6])
     _txt(f"{folder}/NG.txt", [ng], "%d"); _txt(f"{folder}/NN.txt", [2 * ng], "%d")
     _txt(f"{folder}/z0.txt", [z0]); _txt(f"{folder}/r0.txt", [0.9 * Lz]); _txt(f"{folder}/x0.txt", [0.0])
     _txt(f"{folder}/top.txt", [top]); _txt(f"{folder}/angle.txt", [0.0]); _txt(f"{folder}/min_angle.txt", [0.0])

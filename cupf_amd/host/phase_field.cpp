@@ -339,7 +339,14 @@ void PhaseField::dumpSetup(const std::string& dir) const
     P(dt); P(tau0); P(lamd); P(hi); P(cfl); P(d0); P(beta0); P(W0); P(dx); P(lxd); P(lyd); P(lzd);
     P(G); P(R); P(underCoolingRate); P(underCoolingRate0); P(z0); P(top); P(num_theta);
     P(thermalType); P(bcX); P(bcY); P(bcZ); P(pts_cell); P(kin_delta); P(L_cp); P(sqrt2);
+    P(nuc_rad); P(undcool_mean); P(undcool_std); P(nuc_Nmax); P(nuc_Nmax0); P(seed_val);
 #undef P
+    // the macro grid as the kernels receive it (cudaSetup)
+    f << "Nx " << mac.Nx << "\nNy " << mac.Ny << "\nNz " << mac.Nz << "\nNt " << mac.Nt << "\n";
+    if (mac.Nx > 1 && mac.Ny > 0 && mac.Nz > 0 && mac.Nt > 0)
+        f << "X0_mac " << mac.X_mac[0] << "\nY0_mac " << mac.Y_mac[0] << "\nZ0_mac " << mac.Z_mac[0]
+          << "\nt0_mac " << mac.t_mac[0] << "\nDx_mac " << (mac.X_mac[1] - mac.X_mac[0]) << "\nDt_mac "
+          << (mac.Nt > 1 ? mac.t_mac[1] - mac.t_mac[0] : 1.0f) << "\n";
     std::ofstream a(dir + "/init_phi.bin", std::ios::binary);
     a.write((const char*)phi.data(), sizeof(float) * phi.size());
     std::ofstream b(dir + "/init_alpha.bin", std::ios::binary);
@@ -496,11 +503,12 @@ void APTPhaseField::getLineQoIs(MovingDomain* md)
     const float threshold = params.z0 + params.top * (md->samples + 1) / params.nts;
     if (loc > threshold) {
         std::printf("threshold %f \n", threshold);
-        check(pf_collect(ctx_, 0, PF_COLLECT_CLEAR_ALPHA, nullptr, alpha.data()), "pf_collect");
+        // alpha stays on the device: the tip scan and per-grain counts run there
+        check(pf_collect(ctx_, 0, PF_COLLECT_CLEAR_ALPHA, nullptr, nullptr), "pf_collect");
         check(pf_md_download(ctx_, nullptr, md->loss_area_host.data(), z.data()), "pf_md_download");
         md->cur_tip = 0;
-        qois->calculateLineQoIs(params, md->cur_tip, alpha.data(), md->samples + 1, z.data(),
-                                md->loss_area_host.data(), md->move_count);
+        qois->calculateLineQoIsDevice(ctx_, params, md->cur_tip, md->samples + 1, z.data(),
+                                      md->loss_area_host.data(), md->move_count);
         md->samples += 1;
     }
 }
@@ -514,8 +522,9 @@ void APTPhaseField::moveDomain(MovingDomain* md)
         md->tip_front -= 1;
     }
     md->lowsl = 1;
-    check(pf_collect(ctx_, 0, 0, nullptr, alpha.data()), "pf_collect");
-    qois->sampleHeights(md->lowsl, alpha.data(), fnx, fny, fnz);
+    // the tip scan on the device alpha: fnz plane flags cross PCIe, not alpha
+    check(pf_collect(ctx_, 0, 0, nullptr, nullptr), "pf_collect");
+    qois->sampleHeightsDevice(ctx_, md->lowsl);
 }
 
 void APTPhaseField::evolve()
@@ -533,13 +542,14 @@ void APTPhaseField::evolve()
         check(pf_collect(ctx_, 0, PF_COLLECT_CLEAR_ALPHA | PF_COLLECT_NO_BC, nullptr, nullptr), "pf_collect");
         check(pf_reinit_from_collect(ctx_), "pf_reinit_from_collect");
     }
-    check(pf_collect(ctx_, 0, PF_COLLECT_CLEAR_ALPHA | PF_COLLECT_NO_BC, nullptr, alpha.data()), "pf_collect");
+    check(pf_collect(ctx_, 0, PF_COLLECT_CLEAR_ALPHA | PF_COLLECT_NO_BC, nullptr,
+                     ds->useLineConfig ? nullptr : alpha.data()), "pf_collect");
     if (ds->useLineConfig) {
         md.tip_thres = (int)(0.8 * fnz);
         md.loss_area_host.assign(params.num_theta, 0);
         check(pf_md_enable(ctx_, fnz_f), "pf_md_enable");
-        qois->calculateLineQoIs(params, md.cur_tip, alpha.data(), 0, z.data(), md.loss_area_host.data(),
-                                md.move_count);
+        qois->calculateLineQoIsDevice(ctx_, params, md.cur_tip, 0, z.data(), md.loss_area_host.data(),
+                                      md.move_count);
     } else if (!ds->includeNucleation) {
         qois->calculateQoIs(params, alpha.data(), 0);
     }
@@ -587,5 +597,10 @@ void APTPhaseField::evolve()
     long long ovf = 0;
     pf_overflow_count(ctx_, &ovf);
     std::printf("neighbourhoods over %d slots: %lld\n", NUM_PF, ovf);
+    if (ds->includeNucleation) {
+        long long nev = 0;
+        pf_nucl_events(ctx_, &nev);
+        std::printf("nucleation events: %lld\n", nev);
+    }
     params.Mt = 2 * kt;
 }

@@ -138,6 +138,37 @@ void QOI::sampleHeights(int& cur_tip, const int* alpha, int fnx, int fny, int fn
     cur_tip = scan_tip(cur_tip, alpha, fnx, fny, fnz);
 }
 
+// 3DgrainMPI/QOI.cpp:38-97 with the tip scan and the per-grain counts on the
+// device (pf_qoi_line); same results as calculateLineQoIs on the host alpha.
+void QOI::calculateLineQoIsDevice(pf_ctx* ctx, const GlobalConstants& params, int& cur_tip, int kt,
+                                  const float* z, const int* loss_area, int move_count)
+{
+    const int NG = params.num_theta;
+    std::vector<int> t((size_t)NG), e((size_t)NG);
+    int tip = cur_tip;
+    if (pf_qoi_line(ctx, cur_tip, &tip, t.data(), e.data()) != PF_OK)
+        throw std::runtime_error(std::string("pf_qoi_line: ") + pf_last_error());
+    cur_tip = tip;
+    auto& tip_y = mQoIVectorIntData["tip_y"];
+    if (kt >= 0 && kt < (int)tip_y.size()) tip_y[kt] = cur_tip + move_count;
+    std::printf("frame %d, ntip %d, tip %f\n", kt, cur_tip + move_count, z[cur_tip]);
+    auto& total = mQoIVectorIntData["total_area"];
+    auto& extra = mQoIVectorIntData["extra_area"];
+    for (int g = 0; g < NG; ++g) {
+        total[(size_t)kt * NG + g] += t[g] + loss_area[g];
+        extra[(size_t)kt * NG + g] += e[g];
+    }
+}
+
+// 3DgrainMPI/QOI.cpp:162-185 on the device alpha (pf_qoi_tip).
+void QOI::sampleHeightsDevice(pf_ctx* ctx, int& cur_tip)
+{
+    int tip = cur_tip;
+    if (pf_qoi_tip(ctx, cur_tip, &tip) != PF_OK)
+        throw std::runtime_error(std::string("pf_qoi_tip: ") + pf_last_error());
+    cur_tip = tip;
+}
+
 // 3DgrainMPI/QOI.cpp:103-160
 void QOILine::searchJunctionsOnImage(const GlobalConstants& params, const int* alpha)
 {

@@ -23,6 +23,11 @@ public:
     // the caller then downloads alpha and uses calculateQoIs.
     bool calculateQoIsDevice(pf_ctx* ctx, const GlobalConstants& params);
     void sampleHeights(int& cur_tip, const int* alpha, int fnx, int fny, int fnz);
+    // calculateLineQoIs and sampleHeights on the context's device alpha (the
+    // last pf_collect): only plane flags and per-grain counts cross PCIe.
+    void calculateLineQoIsDevice(pf_ctx* ctx, const GlobalConstants& params, int& cur_tip, int kt,
+                                 const float* z, const int* loss_area, int move_count);
+    void sampleHeightsDevice(pf_ctx* ctx, int& cur_tip);
     std::map<std::string, std::vector<int>> mQoIVectorIntData;
     std::map<std::string, std::vector<float>> mQoIVectorFloatData;
     int mNumActiveGrains = 0;

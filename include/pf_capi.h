@@ -190,6 +190,13 @@ int pf_qoi_junctions3d(pf_ctx* ctx, int* rows, int cap, int* count);
  * moving domain's loss area to total. */
 int pf_qoi_line(pf_ctx* ctx, int cur_tip, int* new_tip, int* total, int* extra);
 
+/* QOI::sampleHeights (3DgrainMPI/QOI.cpp:162-185), called by moveDomain
+ * (3DgrainMPI/APTPhaseField.cu:696): *new_tip = the first plane above
+ * cur_tip whose interior holds a liquid point (alpha 0) on the collected
+ * alpha, minus one (fnz - 2 if none up to fnz - 2).  Only fnz plane flags
+ * cross PCIe instead of the whole alpha field. */
+int pf_qoi_tip(pf_ctx* ctx, int cur_tip, int* new_tip);
+
 /* Liquid-start re-initialisation (3DgrainMPI/APTPhaseField.cu:809-818): both
  * buffers set to -1, then slot 0 = (initial phi0, collected alpha) where
  * phi0 > -0.995.  Follows a pf_collect with PF_COLLECT_CLEAR_ALPHA. */

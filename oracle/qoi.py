@@ -6,6 +6,14 @@ alpha > 0 in first-seen (k, j, i) scan order (:198-231); 3-D junctions: solid
 interior points whose 19-point neighbourhood holds no liquid and exactly 4
 distinct ids with the largest count <= 10 (:233-311); ids compared as sorted
 sets (the reference's order comes from unordered_map iteration).
+2-D junctions of the line configuration on the tip planes of the archived
+alpha (QOILine::searchJunctionsOnImage, :103-160): interior points of each
+frame's tip plane whose 3x3 window holds at least 3 distinct values (liquid
+0 included) with the largest count <= 5; the values are written ascending
+(the reference's order comes from unordered_map iteration), at most 5 of
+them (the reference's row has room for 5 and writes a 6th into the next
+row), and rows past a frame's share of node_region are skipped (the
+reference writes into the next frame's share).
 """
 import numpy as np
 
@@ -56,3 +64,32 @@ def junctions3d(alpha, fnx, fny, fnz):
         if ids.size == 4 and cnt.max() <= 10:
             out.append((int(i + 1), int(j + 1), int(k + 1), int(cnt.max()), tuple(int(x) for x in ids)))
     return out
+
+
+def line_junctions(alpha_full, fnx, fny, tip_y, num_nodes, nfeat=8, repeated=5):
+    """QOILine::searchJunctionsOnImage (3DgrainMPI/QOI.cpp:103-160) on the
+    full-height alpha (fnz_full planes of fnx * fny, halo included) at the
+    frames' tip planes tip_y.  Returns (cross_sec, node_region): the tip
+    planes one after another, and rows of nfeat ints (i, j, largest count,
+    then the ids) in each frame's share of repeated * num_nodes rows, -1
+    elsewhere."""
+    A = np.asarray(alpha_full).reshape(-1, fny, fnx)
+    per_frame = repeated * num_nodes * nfeat
+    nodes = np.full(len(tip_y) * per_frame, -1, np.int32)
+    cross = []
+    for kt, tip in enumerate(tip_y):
+        P = A[int(tip)]
+        cross.append(P.ravel().astype(np.int32))
+        cnt = 0
+        for j in range(1, fny - 1):
+            for i in range(1, fnx - 1):
+                ids, c = np.unique(P[j - 1:j + 2, i - 1:i + 2], return_counts=True)
+                if ids.size >= 3 and c.max() <= 5:
+                    base = per_frame * kt + cnt * nfeat
+                    if base + nfeat > per_frame * (kt + 1):
+                        continue
+                    nodes[base:base + 3] = (i, j, int(c.max()))
+                    for q, v in enumerate(ids[:nfeat - 3]):
+                        nodes[base + 3 + q] = int(v)
+                    cnt += 1
+    return np.concatenate(cross) if cross else np.zeros(0, np.int32), nodes

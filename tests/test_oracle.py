@@ -686,3 +686,67 @@ def test_volume_and_junction_restatements_against_loops_and_a_hand_case():
                                             for jj in (2, 3) for ii in (2, 3)]
     assert qoi.junctions3d(H.ravel(), 6, 6, 6) == want
     assert qoi.volume(H.ravel(), 6, 6, 6) == ([1, 2, 3, 4], [15, 16, 16, 16])
+
+
+def _line_junction_loops(A, tip_y, num_nodes, nfeat=8, repeated=5):
+    """QOILine::searchJunctionsOnImage (3DgrainMPI/QOI.cpp:103-160) as loops
+    with a dict count, the decisions of this build marked: ids ascending, at
+    most nfeat - 3 of them, rows past a frame's share skipped."""
+    _, fny, fnx = A.shape
+    per = repeated * num_nodes * nfeat
+    nodes = [-1] * (len(tip_y) * per)
+    cross = []
+    for kt, tip in enumerate(tip_y):
+        cross.extend(int(v) for v in A[tip].ravel())
+        node_cnt = 0
+        for j in range(1, fny - 1):
+            for i in range(1, fnx - 1):
+                occur = {}
+                for dj in (-1, 0, 1):
+                    for di in (-1, 0, 1):
+                        a = int(A[tip, j + dj, i + di])
+                        occur[a] = occur.get(a, 0) + 1
+                if len(occur) >= 3 and max(occur.values()) <= 5:
+                    off = per * kt + node_cnt * nfeat
+                    if off + nfeat > per * (kt + 1):      # decision: no write past the frame
+                        continue
+                    nodes[off], nodes[off + 1], nodes[off + 2] = i, j, max(occur.values())
+                    for q, a in enumerate(sorted(occur)[:nfeat - 3]):   # decision: ascending, <= 5
+                        nodes[off + 3 + q] = a
+                    node_cnt += 1
+    return cross, nodes
+
+
+def test_line_junction_restatement_against_loops_and_a_hand_case():
+    """oracle/qoi.py::line_junctions against the loops above on random tip
+    planes (3-9 values per window, liquid included, frames that overflow
+    their share), and a hand-worked plane of four quadrant grains."""
+    from oracle import qoi
+    rng = np.random.default_rng(4)
+    seen = over = wide = 0
+    for trial in range(6):
+        fnx, fny, nz = 9 + trial, 7, 12
+        b = 1 + trial % 3
+        jj, ii = np.meshgrid(np.arange(fny), np.arange(fnx), indexing="ij")
+        A = np.stack([rng.integers(0, 7, size=(fny // b + 1, fnx // b + 1))[jj // b, ii // b]
+                      for _ in range(nz)]).astype(np.int32)
+        tips = [int(t) for t in rng.integers(0, nz, size=3)]
+        nn = 1 + trial % 2
+        cross, nodes = qoi.line_junctions(A.ravel(), fnx, fny, tips, nn)
+        c2, n2 = _line_junction_loops(A, tips, nn)
+        assert cross.tolist() == c2 and nodes.tolist() == n2
+        rows = np.asarray(n2).reshape(-1, 8)
+        seen += int((rows[:, 0] != -1).sum())
+        over += int(all((rows[q * 5 * nn:(q + 1) * 5 * nn, 0] != -1).all() for q in range(3)))
+        wide += int((rows[:, 7] != -1).sum())
+    assert seen > 30 and over > 0 and wide > 0
+    # hand case: a 6x6 plane (4x4 interior) of four quadrant grains split at
+    # i >= 3 and j >= 3.  The 3x3 windows at (2|3, 2|3) hold all four ids,
+    # one of them 4 times; every other interior window holds at most 2 ids.
+    jj, ii = np.meshgrid(np.arange(6), np.arange(6), indexing="ij")
+    H = (1 + (ii >= 3) + 2 * (jj >= 3)).astype(np.int32)[None]
+    cross, nodes = qoi.line_junctions(H.ravel(), 6, 6, [0], 1)
+    rows = nodes.reshape(-1, 8)
+    assert rows[:4].tolist() == [[2, 2, 4, 1, 2, 3, 4, -1], [3, 2, 4, 1, 2, 3, 4, -1],
+                                 [2, 3, 4, 1, 2, 3, 4, -1], [3, 3, 4, 1, 2, 3, 4, -1]]
+    assert (rows[4:] == -1).all() and cross.tolist() == H.ravel().tolist()
