@@ -933,6 +933,58 @@ __device__ __forceinline__ void scan_all_lds(const int* pD, const int* pC, const
     }
 }
 
+// Overflow policy (above) for a 7-point point inside the tile, from the LDS
+// ring, given the normal gather g of the point (g.n == K, g.ovf set): its slot
+// list holds the first K distinct ids in first-appearance order.  If every
+// entry outside that list has a smaller phi than the smallest of the list
+// ids' largest phi, every other id ranks below all K list ids, so the list is
+// the kept set, in the policy's (first-appearance) order, and g stands as it
+// is.  A list id's largest phi is taken over the entries its row nibbles name;
+// an entry a nibble does not name (another id, or a second entry of a list id
+// in one row) counts as outside, which can only make the test fail, never pass
+// wrongly.  Returns false when the test fails: the fix-up kernel then applies
+// the policy in full.  (At the 256x256x512 start-up, 83-96 % of the overflowed
+// points pass.)
+template <int K>
+__device__ __forceinline__ bool overflow_keeps_list(const int* iD, const int* iC, const int* iU,
+                                                    const float* fD, const float* fC,
+                                                    const float* fU, int pt, const Gather<K>& g)
+{
+    auto off = [&](int s) {
+        return s == 1 ? pt - kHX : s == 2 ? pt - 1 : s == 4 ? pt + 1 : s == 5 ? pt + kHX : pt;
+    };
+    float best[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) best[q] = -__builtin_huge_valf();
+    float outside = -__builtin_huge_valf();
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+        const int* ib = s == 0 ? iD : (s == 6 ? iU : iC);
+        const float* fb = s == 0 ? fD : (s == 6 ? fU : fC);
+        const int o = off(s);
+        uint32_t named = 0u;  // neighbour slots a nibble of this row names
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const uint32_t p = (g.pm[s] >> (4 * q)) & 15u;
+            if (p == 15u) continue;
+            named |= 1u << p;
+            const float v = fb[(int)p * kNPT + o];
+            best[q] = v > best[q] ? v : best[q];
+        }
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+            if ((named >> p) & 1u) continue;
+            if (ib[p * kNPT + o] == -1) continue;
+            const float v = fb[p * kNPT + o];
+            outside = v > outside ? v : outside;
+        }
+    }
+    float low = __builtin_huge_valf();
+#pragma unroll
+    for (int q = 0; q < K; ++q) low = best[q] < low ? best[q] : low;
+    return outside < low;
+}
+
 // Append the overflowed lanes of a wave to the fix-up list (one atomic per wave).
 __device__ __forceinline__ void fixup_append(const StepArgs& a, int ovf, unsigned point)
 {
@@ -960,7 +1012,7 @@ __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld
                                            float (*rph)[K * kNPT], const float* zs, int i, int j,
                                            int k, int k0, int tile, int pt, bool lane_on, float xi,
                                            float yj, float a_s, float epsv, bool stage_next,
-                                           bool act_next2)
+                                           bool act_next2, unsigned& nres)
 {
     const int A = a.fnx * a.fny;
     __syncthreads();
@@ -971,6 +1023,18 @@ __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld
     if (!lane_on) {  // lane past the end of a partial tile: nothing to compute or store
         g.n = 0;
         g.ovf = false;
+    }
+    // 7-point overflow settled here from the ring when the gather's list is
+    // the kept set (the wave skips this when no lane overflowed); the rest go
+    // to the fix-up kernel.  nres counts the wave's settled points for the
+    // overflow diagnostic.
+    if (!G27) {
+        const bool was = g.ovf;
+        if (__ballot(was) != 0ull) {
+            if (was && overflow_keeps_list<K>(rid[sD], rid[sC], rid[sU], rph[sD], rph[sC], rph[sU], pt, g))
+                g.ovf = false;
+            nres += (unsigned)__popcll(__ballot(was && !g.ovf));
+        }
     }
     const int ovf = g.ovf ? 1 : 0;
     const int C = i + j * a.fnx + k * A;
@@ -1094,9 +1158,11 @@ __device__ __forceinline__ void plan_wave(const StepArgs& a, int gw, int lane, b
     }
 }
 
+// nres (wave-uniform) accumulates the overflowed points the tile resolved.
 template <int K, bool G27>
 __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, int k0, int k1,
-                                           uint64_t need, uint64_t act, float a_s, float epsv)
+                                           uint64_t need, uint64_t act, float a_s, float epsv,
+                                           unsigned& nres)
 {
     __shared__ int rid[3][K * kNPT];
     __shared__ float rph[3][K * kNPT];
@@ -1134,7 +1200,7 @@ __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, in
         __builtin_amdgcn_s_waitcnt(0);  // the three planes are in LDS (barrier in tile_plane)
         do {
             tile_plane<K, G27>(a, ld, rid, rph, zs, i, j, k, k0, tile, pt, lane_on, xi, yj, a_s,
-                               epsv, need_at(k + 1), act_at(k + 2));
+                               epsv, need_at(k + 1), act_at(k + 2), nres);
             ++k;
         } while (need_at(k));
     }

@@ -19,6 +19,7 @@
 
 #include "../../include/pf_capi.h"
 #include "apt_internal.h"
+#include "capacity.h"
 
 using namespace cupf;
 
@@ -51,6 +52,8 @@ struct pf_ctx {
     float* ball = nullptr;
     int cnx = 0, cny = 0, cnz = 0;
     unsigned long long ncall = 0;
+    long long wl_alloc = 0;         // entries allocated for wl_item / wl_need / wl_act
+    long long fix_alloc = 0;        // entries allocated for fix_list
     int kchunk = 4;                 // z planes per work item
     // Activity map per buffer (one word per 64x4 column tile and z plane) and
     // the epoch each buffer was last written at; see StepArgs.
@@ -567,11 +570,11 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         (rc = dalloc(&c->zbuf, p.fnz)) || (rc = dalloc(&c->phi0, (size_t)L)) ||
         (rc = dalloc(&c->phi, (size_t)L)) || (rc = dalloc(&c->alpha, (size_t)L)) ||
         (rc = dalloc(&c->counters, 2)) || (rc = dalloc(&c->sums, p.fnz)) ||
-        (rc = dalloc(&c->fix_list, (size_t)(p.fnx - 2) * (p.fny - 2) * (p.fnz - 2))) ||
+        (rc = dalloc(&c->fix_list, (size_t)(c->fix_alloc = (long long)(p.fnx - 2) * (p.fny - 2) * (p.fnz - 2)))) ||
         (rc = dalloc(&c->fix_count, 8)) || (rc = dalloc(&c->sink, 128)) ||
         (rc = dalloc(&c->act[0], (size_t)c->ntiles * p.fnz)) ||
         (rc = dalloc(&c->act[1], (size_t)c->ntiles * p.fnz)) ||
-        (rc = dalloc(&c->wl_item, (size_t)c->nzchunk * c->items_chunk + 1)) ||
+        (rc = dalloc(&c->wl_item, (size_t)(c->wl_alloc = (long long)c->nzchunk * c->items_chunk + 1))) ||
         (rc = dalloc(&c->wl_need, (size_t)c->nzchunk * c->items_chunk + 1)) ||
         (rc = dalloc(&c->wl_act, (size_t)c->nzchunk * c->items_chunk + 1)))
         return bail(rc);
@@ -754,10 +757,25 @@ int pf_apply_bc(pf_ctx* c, int which)
     return PF_OK;
 }
 
+// The step's lists hold their worst case, recomputed from the grid and the
+// chunk depth (capacity.h), and the derived sizes the kernels index with
+// agree with it; otherwise nothing is launched.
+static int check_capacity(const pf_ctx* c)
+{
+    const pf_params& p = c->p;
+    const long long items = worst_list_items(p.fnx, p.fny, p.fnz, c->kchunk, kPartialMin);
+    if ((long long)c->nzchunk * c->items_chunk != items || items > c->wl_alloc ||
+        worst_fix_entries(p.fnx, p.fny, p.fnz) > c->fix_alloc)
+        return fail(PF_E_STATE, "work list capacity below its worst case (" + std::to_string(items) +
+                                    " chunks, " + std::to_string(c->wl_alloc) + " allocated)");
+    return PF_OK;
+}
+
 static int step_impl(pf_ctx* c, int nsteps, int first_step, int flags)
 {
     if (!c->have_coords || !c->have_orient || !c->have_state)
         return fail(PF_E_STATE, "pf_step needs coordinates, orientations and an initial state");
+    if (int rc = check_capacity(c)) return rc;
     const pf_params& p = c->p;
     if (p.thermal_type == PF_THERMAL_PARABOLOID && !c->psi_mac)
         return fail(PF_E_STATE, "paraboloid thermal field needs psi_mac");
@@ -1327,6 +1345,16 @@ int pf_batch_step(pf_batch* b, int nsteps, const int* first_step, int flags)
             return fail(PF_E_STATE, "batch member needs coordinates, orientations and a state");
         if (c->p.thermal_type == PF_THERMAL_PARABOLOID && !c->psi_mac)
             return fail(PF_E_STATE, "paraboloid thermal field needs psi_mac");
+    }
+    {
+        // the batch list holds every member's worst case at any depth >= kMinChunk
+        long long need = 0;
+        for (pf_ctx* c : b->ctxs) {
+            if (int rc = check_capacity(c)) return rc;
+            need += worst_list_items(c->p.fnx, c->p.fny, c->p.fnz, pf_batch::kMinChunk, kPartialMin);
+        }
+        if (need > (long long)b->wl_cap)
+            return fail(PF_E_STATE, "batch work list capacity below its worst case");
     }
     HIPCHK(hipSetDevice(b->device));
     const int use_iuc = (flags & PF_STEP_INITIAL_UC) ? 1 : 0;

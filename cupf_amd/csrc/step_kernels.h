@@ -26,6 +26,7 @@ __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
     const unsigned nh = __atomic_load_n(a.wl_count, __ATOMIC_RELAXED);
     const unsigned n = ne + nh + __atomic_load_n(a.wl_light, __ATOMIC_RELAXED);
     const unsigned cap = (unsigned)(a.nzchunk * a.items_chunk);
+    unsigned nres = 0u;  // overflowed points this wave resolved in its tiles
     for (;;) {
         // the next item is taken only when this workgroup is free, which keeps
         // the tail of the list balanced
@@ -53,9 +54,10 @@ __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
             const int tx = t - ty * a.tiles_x;
             const int k0 = 1 + zc * a.kchunk;
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-            tile_march<K, G27>(a, tx, ty, k0, k1, a.wl_need[w], a.wl_act[w], a_s, epsv);
+            tile_march<K, G27>(a, tx, ty, k0, k1, a.wl_need[w], a.wl_act[w], a_s, epsv, nres);
         }
     }
+    if ((threadIdx.x & 63) == 0 && nres) atomicAdd(a.overflow, (unsigned long long)nres);
 }
 
 // Fix-up pass for the points of the last apt_step launch whose neighbourhood
@@ -156,7 +158,9 @@ __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
             const int tx = t - ty * a.tiles_x;
             const int k0 = 1 + zc * a.kchunk;
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-            tile_march<K, G27>(a, tx, ty, k0, k1, b.wl_need[w], b.wl_act[w], a_s, epsv);
+            unsigned nres = 0u;
+            tile_march<K, G27>(a, tx, ty, k0, k1, b.wl_need[w], b.wl_act[w], a_s, epsv, nres);
+            if ((threadIdx.x & 63) == 0 && nres) atomicAdd(a.overflow, (unsigned long long)nres);
         }
         __syncthreads();  // s_q is rewritten for the next item
     }
