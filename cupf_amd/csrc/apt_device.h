@@ -574,16 +574,18 @@ __device__ __forceinline__ void topk_push(TopK<K>& c, int id, float v, int pos)
     c.n += full ? 0 : 1;
 }
 
-// Candidates of one neighbour point; P is its position in the scan order.
+// Candidates of one neighbour point, entry p at arg/ph[base + p * L]; P is
+// the point's position in the scan order.
 template <int K>
-__device__ __forceinline__ void topk_point(const StepArgs& a, int base, int P, TopK<K>& c)
+__device__ __forceinline__ void topk_point(const int* __restrict__ arg, const float* __restrict__ ph,
+                                           int base, int L, int P, TopK<K>& c)
 {
     int ids[K];  // all 2K loads first, as in scan_point
     float phs[K];
 #pragma unroll
     for (int p = 0; p < K; ++p) {
-        ids[p] = a.arg[base + p * a.L];
-        phs[p] = a.ph[base + p * a.L];
+        ids[p] = arg[base + p * L];
+        phs[p] = ph[base + p * L];
     }
 #pragma unroll
     for (int p = 0; p < K; ++p) {
@@ -646,13 +648,13 @@ __device__ __forceinline__ void gather_overflow(const StepArgs& a, const Nbr& nb
     }
     gather_init(g);
     if (!G27) {
-        topk_point<K>(a, nbase(a, nb, 1, 1, 0), 0, c);
-        topk_point<K>(a, nbase(a, nb, 1, 0, 1), 1, c);
-        topk_point<K>(a, nbase(a, nb, 0, 1, 1), 2, c);
-        topk_point<K>(a, nbase(a, nb, 1, 1, 1), 3, c);
-        topk_point<K>(a, nbase(a, nb, 2, 1, 1), 4, c);
-        topk_point<K>(a, nbase(a, nb, 1, 2, 1), 5, c);
-        topk_point<K>(a, nbase(a, nb, 1, 1, 2), 6, c);
+        topk_point<K>(a.arg, a.ph, nbase(a, nb, 1, 1, 0), a.L, 0, c);
+        topk_point<K>(a.arg, a.ph, nbase(a, nb, 1, 0, 1), a.L, 1, c);
+        topk_point<K>(a.arg, a.ph, nbase(a, nb, 0, 1, 1), a.L, 2, c);
+        topk_point<K>(a.arg, a.ph, nbase(a, nb, 1, 1, 1), a.L, 3, c);
+        topk_point<K>(a.arg, a.ph, nbase(a, nb, 2, 1, 1), a.L, 4, c);
+        topk_point<K>(a.arg, a.ph, nbase(a, nb, 1, 2, 1), a.L, 5, c);
+        topk_point<K>(a.arg, a.ph, nbase(a, nb, 1, 1, 2), a.L, 6, c);
         scan_point_kept<K, 0>(a.arg, nbase(a, nb, 1, 1, 0), a.L, g, c);
         scan_point_kept<K, 1>(a.arg, nbase(a, nb, 1, 0, 1), a.L, g, c);
         scan_point_kept<K, 2>(a.arg, nbase(a, nb, 0, 1, 1), a.L, g, c);
@@ -662,7 +664,7 @@ __device__ __forceinline__ void gather_overflow(const StepArgs& a, const Nbr& nb
         scan_point_kept<K, 6>(a.arg, nbase(a, nb, 1, 1, 2), a.L, g, c);
     } else {
 #pragma unroll 1
-        for (int q = 0; q < 27; ++q) topk_point<K>(a, nbase_q(a, nb, q), q, c);
+        for (int q = 0; q < 27; ++q) topk_point<K>(a.arg, a.ph, nbase_q(a, nb, q), a.L, q, c);
         kept_run<K>(a, nb, 0, 4, g, c);
         scan_point_kept<K, 0>(a.arg, nbase_q(a, nb, 4), a.L, g, c);
         kept_run<K>(a, nb, 5, 10, g, c);
@@ -677,6 +679,52 @@ __device__ __forceinline__ void gather_overflow(const StepArgs& a, const Nbr& nb
         scan_point_kept<K, 6>(a.arg, nbase_q(a, nb, 22), a.L, g, c);
         kept_run<K>(a, nb, 23, 27, g, c);
     }
+}
+
+// The record buffer named by the fix-up list's header (kFixHeader): entry
+// f = s*K + p of record q has its id at id[f * cap + q] and its phi at
+// ph[f * cap + q]; ph follows the 7 * K * cap ids in the same allocation.
+struct RecView {
+    int* id;
+    float* ph;
+    int cap;
+};
+
+template <int K>
+__device__ __forceinline__ RecView rec_view(const StepArgs& a)
+{
+    RecView r;
+    r.cap = (int)a.fix_list[-4];
+    r.id = *reinterpret_cast<int* const*>(a.fix_list - 2);
+    r.ph = reinterpret_cast<float*>(r.id + (size_t)7 * K * (size_t)r.cap);
+    return r;
+}
+
+// The same policy on a fix-up record: the point's 7 x K entries in scan order
+// (D,B,L,C,R,T,U, then slot), entry f = s*K + p at rid/rph[f * cap + q]
+// (written by the tile from its LDS ring, see rec_store).
+template <int K>
+__device__ __forceinline__ void gather_overflow_rec(const int* rid, const float* rph, int cap, int q,
+                                                    Gather<K>& g)
+{
+    TopK<K> c;
+    c.n = 0;
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+        c.id[u] = -1;
+        c.mx[u] = 0.0f;
+        c.pos[u] = 0;
+    }
+    gather_init(g);
+#pragma unroll 1
+    for (int s = 0; s < 7; ++s) topk_point<K>(rid + s * K * cap, rph + s * K * cap, q, cap, s, c);
+    scan_point_kept<K, 0>(rid + 0 * K * cap, q, cap, g, c);
+    scan_point_kept<K, 1>(rid + 1 * K * cap, q, cap, g, c);
+    scan_point_kept<K, 2>(rid + 2 * K * cap, q, cap, g, c);
+    scan_point_kept<K, 3>(rid + 3 * K * cap, q, cap, g, c);
+    scan_point_kept<K, 4>(rid + 4 * K * cap, q, cap, g, c);
+    scan_point_kept<K, 5>(rid + 5 * K * cap, q, cap, g, c);
+    scan_point_kept<K, 6>(rid + 6 * K * cap, q, cap, g, c);
 }
 
 // One interior point: pass A, then pass B.  Returns 0 if no neighbour holds a
@@ -696,6 +744,35 @@ __device__ __forceinline__ int apt_point(const StepArgs& a, int i, int j, int k,
     stencil_bases(a, nb, b);
     rhs_point<K>(a, i, j, k, b, g, a_s, epsv);
     return g.n > 0 ? 1 : 0;
+}
+
+// Overflowed fix-up list entry q (fix-up kernels only): point index, and
+// whether the tile left the point's entries in the record buffer.
+__device__ __forceinline__ void fixup_point(const StepArgs& a, unsigned q, int& i, int& j, int& k,
+                                            bool& rec)
+{
+    const unsigned e = a.fix_list[q];
+    rec = (e & kRecFlag) != 0u;
+    const int C = (int)(e & ~kRecFlag);
+    const int A = a.fnx * a.fny;
+    k = C / A;
+    const int r = C - k * A;
+    j = r / a.fnx;
+    i = r - j * a.fnx;
+}
+
+// The overflow policy and pass B for a point whose entries are in record q.
+template <int K>
+__device__ __forceinline__ void apt_point_overflow_rec(const StepArgs& a, int i, int j, int k, int q,
+                                                       float a_s, float epsv)
+{
+    const RecView r = rec_view<K>(a);
+    Gather<K> g;
+    gather_overflow_rec<K>(r.id, r.ph, r.cap, q, g);
+    const int C = i + j * a.fnx + k * a.fnx * a.fny;
+    rhs_point<K>(a, i, j, k, C, g, a_s, epsv, [&](int s, uint32_t p) {
+        return r.ph[(s * K + (int)p) * r.cap + q];
+    });
 }
 
 // The same point under the overflow policy (fix-up kernel only).
@@ -985,17 +1062,50 @@ __device__ __forceinline__ bool overflow_keeps_list(const int* iD, const int* iC
     return outside < low;
 }
 
+constexpr unsigned kNoRec = 0xFFFFFFFFu;
+
 // Append the overflowed lanes of a wave to the fix-up list (one atomic per wave).
-__device__ __forceinline__ void fixup_append(const StepArgs& a, int ovf, unsigned point)
+// With `rec`, an entry whose position has a record slot (below the record
+// capacity in the list's header) is flagged and its position returned, for
+// the caller to write the record there; kNoRec otherwise.
+__device__ __forceinline__ unsigned fixup_append(const StepArgs& a, int ovf, unsigned point,
+                                                 bool rec = false)
 {
     const unsigned long long m = __ballot(ovf);
-    if (m == 0ull) return;
+    if (m == 0ull) return kNoRec;
     const int lane = threadIdx.x & 63;
     const int leader = __ffsll((long long)m) - 1;
     unsigned base = 0;
     if (lane == leader) base = atomicAdd(a.fix_count, (unsigned)__popcll(m));
     base = __shfl(base, leader);
-    if (ovf) a.fix_list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = point;
+    const unsigned pos = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    const unsigned cap = rec ? a.fix_list[-4] : 0u;
+    const bool has = ovf && pos < cap;
+    if (ovf) a.fix_list[pos] = point | (has ? kRecFlag : 0u);
+    return has ? pos : kNoRec;
+}
+
+// Record of an overflowed point for the fix-up: its 7 x K (id, phi) entries
+// from the LDS ring, in scan order, at record pos (structure of arrays, so the
+// wave's stores and the fix-up's loads are coalesced over consecutive entries).
+template <int K>
+__device__ __forceinline__ void rec_store(const StepArgs& a, const int* iD, const int* iC,
+                                          const int* iU, const float* fD, const float* fC,
+                                          const float* fU, int pt, unsigned pos)
+{
+    const RecView r = rec_view<K>(a);
+    // one row at a time: only 2K values in flight, not 14K registers' worth
+#pragma unroll 1
+    for (int s = 0; s < 7; ++s) {
+        const int o = s == 1 ? pt - kHX : s == 2 ? pt - 1 : s == 4 ? pt + 1 : s == 5 ? pt + kHX : pt;
+        const int* ib = s == 0 ? iD : (s == 6 ? iU : iC);
+        const float* fb = s == 0 ? fD : (s == 6 ? fU : fC);
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+            r.id[(s * K + p) * r.cap + (int)pos] = ib[p * kNPT + o];
+            r.ph[(s * K + p) * r.cap + (int)pos] = fb[p * kNPT + o];
+        }
+    }
 }
 
 // One z plane of a tile.  On entry the ring holds planes k-1, k, k+1.
@@ -1075,8 +1185,11 @@ __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld
     // A point that gathered an id may leave non-empty slots (an overflowed one
     // will, through the fix-up): mark the destination tile plane.
     if (ovf || g.n > 0) a.act_dst[k * a.ntiles + tile] = a.ep_new;
-    // Overflowed lanes are redone after the step by the per-point fix-up.
-    fixup_append(a, ovf, (unsigned)C);
+    // Overflowed lanes are redone after the step by the per-point fix-up, from
+    // a record of their entries where one fits (7-point gather).
+    const unsigned pos = fixup_append(a, ovf, (unsigned)C, !G27);
+    if (!G27 && pos != kNoRec)
+        rec_store<K>(a, rid[sD], rid[sC], rid[sU], rph[sD], rph[sC], rph[sU], pt, pos);
     if (stage_next) {
         __syncthreads();
         loader_store(ld, rid[sD], rph[sD]);

@@ -20,6 +20,17 @@ constexpr double kKill = -0.9999;
 constexpr float kLSf = -0.995f;
 constexpr float kKillf = -0.9999f;
 
+// Fix-up list entry flag: the point's entries are in the record buffer at the
+// entry's position (point indices stay below 2^31 since K*L does).
+constexpr unsigned kRecFlag = 0x80000000u;
+
+// The fix-up list is allocated with kFixHeader words in front of entry 0.
+// Words -4 .. -1 describe the record buffer: record capacity, 0, and the
+// buffer's address (low, high word).  The step kernel reads them from memory
+// only in waves that hand a point to the fix-up, so they cost it no uniform
+// registers (it is at its scalar register limit).
+constexpr int kFixHeader = 64;
+
 // Leftover columns at the +x end narrower than this take the direct path;
 // wider ones form a partial LDS tile (its idle lanes cost less than the direct
 // path's per-point neighbour loads).
@@ -38,7 +49,9 @@ struct StepArgs {
     const float* R12;      // orientation rows, 12 floats per grain (9 used)
     const float* psi_mac;  // type-4 thermal field
     unsigned long long* overflow;  // count of points with > K distinct ids
-    unsigned* fix_list;    // interior points left to the fix-up pass (capacity: interior points)
+    unsigned* fix_list;    // interior points left to the fix-up pass (capacity: interior points);
+                           // kRecFlag set: the entries of the point are in record (list position);
+                           // preceded by the record header (kFixHeader)
     // Counters of this step: fix_count[0] entries in fix_list; wl_count (below)
     // points into the same 4-word set.  Consecutive steps alternate between two
     // sets; cnt_next is the other one, cleared by this step's plan kernel.

@@ -40,7 +40,10 @@ struct pf_ctx {
     float* phi = nullptr;           // collect target
     int* alpha = nullptr;           // collect target / initial alpha (alpha_m)
     unsigned long long* counters = nullptr;  // [0] overflow points, [1] nucleation events
+    unsigned* fix_buf = nullptr;    // record header (kFixHeader words), then the fix-up list
     unsigned* fix_list = nullptr;   // overflow fix-up list, one entry per interior point
+    int* rec = nullptr;             // fix-up records: 7 x K ids, then 7 x K phis (structure of arrays)
+    unsigned rec_cap = 0;           // records: min(interior points / 8, 2^20), none for 27-point
     unsigned* fix_count = nullptr;  // two 4-word counter sets, used by alternate steps
     int* sink = nullptr;            // 128 words written by lanes that must not store
     long long* sums = nullptr;
@@ -320,8 +323,8 @@ void free_ctx(pf_ctx* c)
     void* bufs[] = {c->ph[0], c->ph[1], c->arg[0], c->arg[1], c->x, c->y, c->z, c->zbuf,
                     c->R12, c->psi_mac, c->T3d, c->phi0, c->phi, c->alpha, c->counters,
                     c->sums, c->alpha_full, c->loss, c->nstatus, c->ball,
-                    c->fix_list, c->fix_count, c->sink, c->act[0], c->act[1],
-                    c->wl_item, c->wl_need, c->wl_act};
+                    c->fix_buf, c->fix_count, c->sink, c->act[0], c->act[1],
+                    c->wl_item, c->wl_need, c->wl_act, c->rec};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -570,7 +573,7 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         (rc = dalloc(&c->zbuf, p.fnz)) || (rc = dalloc(&c->phi0, (size_t)L)) ||
         (rc = dalloc(&c->phi, (size_t)L)) || (rc = dalloc(&c->alpha, (size_t)L)) ||
         (rc = dalloc(&c->counters, 2)) || (rc = dalloc(&c->sums, p.fnz)) ||
-        (rc = dalloc(&c->fix_list, (size_t)(c->fix_alloc = (long long)(p.fnx - 2) * (p.fny - 2) * (p.fnz - 2)))) ||
+        (rc = dalloc(&c->fix_buf, kFixHeader + (size_t)(c->fix_alloc = (long long)(p.fnx - 2) * (p.fny - 2) * (p.fnz - 2)))) ||
         (rc = dalloc(&c->fix_count, 8)) || (rc = dalloc(&c->sink, 128)) ||
         (rc = dalloc(&c->act[0], (size_t)c->ntiles * p.fnz)) ||
         (rc = dalloc(&c->act[1], (size_t)c->ntiles * p.fnz)) ||
@@ -578,6 +581,17 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         (rc = dalloc(&c->wl_need, (size_t)c->nzchunk * c->items_chunk + 1)) ||
         (rc = dalloc(&c->wl_act, (size_t)c->nzchunk * c->items_chunk + 1)))
         return bail(rc);
+    if (!p.gather27) {
+        // records of the overflowed points the tile hands to the fix-up (the
+        // first positions of the list; later ones are gathered from the state)
+        const long long interior = (long long)(p.fnx - 2) * (p.fny - 2) * (p.fnz - 2);
+        c->rec_cap = (unsigned)std::max(1LL, std::min(interior / 8, 1LL << 20));
+        if ((rc = dalloc(&c->rec, (size_t)c->rec_cap * 7 * p.K * 2))) return bail(rc);
+    }
+    c->fix_list = c->fix_buf + kFixHeader;
+    unsigned hdr[4] = {c->rec_cap, 0u, 0u, 0u};
+    std::memcpy(&hdr[2], &c->rec, sizeof(int*));
+    static_assert(sizeof(int*) == 8, "record header holds a 64-bit address");
     c->h_sums.resize(p.fnz);
     for (int b = 0; b < 2; ++b) {
         if (hipMemsetD32Async((hipDeviceptr_t)c->ph[b], 0xBF800000u, nst, c->stream) != hipSuccess ||
@@ -586,6 +600,7 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
     }
     if (hipMemsetAsync(c->counters, 0, 2 * sizeof(unsigned long long), c->stream) != hipSuccess ||
         hipMemsetAsync(c->fix_count, 0, 8 * sizeof(unsigned), c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->fix_list - 4, hdr, sizeof(hdr), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         hipMemsetAsync(c->alpha, 0, L * sizeof(int), c->stream) != hipSuccess ||
         hipMemsetAsync(c->act[0], 0, (size_t)c->ntiles * p.fnz * sizeof(unsigned), c->stream) != hipSuccess ||
         hipMemsetAsync(c->act[1], 0, (size_t)c->ntiles * p.fnz * sizeof(unsigned), c->stream) != hipSuccess ||

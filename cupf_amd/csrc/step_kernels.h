@@ -68,10 +68,10 @@ template <int K, bool G27>
 __global__ __launch_bounds__(kFixLanes) void apt_fixup(StepArgs a_arg)
 {
     // The arguments are read from an LDS copy: with them in scalar registers
-    // the compiler keeps 229 VGPRs live (2 waves/SIMD); from LDS it needs 73
-    // (6 waves/SIMD), and each entry is a long chain of dependent loads that
-    // only more waves in flight can overlap (6 workgroups per CU make the
-    // start-up transient of the 256x256x512 run 14 % faster than 2).
+    // the compiler keeps 229 VGPRs live (2 waves/SIMD); from LDS it needs 92
+    // for K=8 (5 waves/SIMD), and each entry is a long chain of dependent
+    // loads that only more waves in flight can overlap (more workgroups per CU
+    // made the start-up transient of the 256x256x512 run 14 % faster than 2).
     __shared__ StepArgs s_a;
     constexpr int kWords = (int)(sizeof(StepArgs) / 4);
     static_assert(sizeof(StepArgs) % 4 == 0 && kWords <= kFixLanes, "StepArgs staging");
@@ -85,15 +85,13 @@ __global__ __launch_bounds__(kFixLanes) void apt_fixup(StepArgs a_arg)
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.overflow, (unsigned long long)n);
     const float a_s = 1.0f + 3.0f * a.kin_delta;
     const float epsv = -4.0f * a.kin_delta / a_s;
-    const int A = a.fnx * a.fny;
 #pragma unroll 1
     for (unsigned q = blockIdx.x * kFixLanes + threadIdx.x; q < n; q += gridDim.x * kFixLanes) {
-        const int C = (int)a.fix_list[q];
-        const int k = C / A;
-        const int r = C - k * A;
-        const int j = r / a.fnx;
-        const int i = r - j * a.fnx;
-        apt_point_overflow<K, G27>(a, i, j, k, a_s, epsv);
+        int i, j, k;
+        bool rec;
+        fixup_point(a, q, i, j, k, rec);
+        if (!G27 && rec) apt_point_overflow_rec<K>(a, i, j, k, (int)q, a_s, epsv);
+        else apt_point_overflow<K, G27>(a, i, j, k, a_s, epsv);
     }
 }
 
@@ -194,15 +192,14 @@ __global__ __launch_bounds__(kFixLanes) void batch_fixup(BatchArgs b)
             else hi = mid - 1;
         }
         const StepArgs& a = b.sims[lo].a;
-        const int C = (int)a.fix_list[g - s_pre[lo]];
-        const int A = a.fnx * a.fny;
-        const int k = C / A;
-        const int r = C - k * A;
-        const int j = r / a.fnx;
-        const int i = r - j * a.fnx;
+        const unsigned q = g - s_pre[lo];
+        int i, j, k;
+        bool rec;
+        fixup_point(a, q, i, j, k, rec);
         const float a_s = 1.0f + 3.0f * a.kin_delta;
         const float epsv = -4.0f * a.kin_delta / a_s;
-        apt_point_overflow<K, G27>(a, i, j, k, a_s, epsv);
+        if (!G27 && rec) apt_point_overflow_rec<K>(a, i, j, k, (int)q, a_s, epsv);
+        else apt_point_overflow<K, G27>(a, i, j, k, a_s, epsv);
     }
 }
 

@@ -41,10 +41,13 @@ class SimSpec:
 
     def footprint(self, K: int = 8) -> int:
         """Device bytes of a context: two K-slot buffers (phi + id), the single
-        fields, the fix-up list and the coordinate tables."""
+        fields, the fix-up list, its records (7 x K id and phi entries for up
+        to min(interior / 8, 2^20) points) and the coordinate tables."""
         fnx, fny, fnz = self.n[0] + 3, self.n[1] + 3, self.n[2] + 3
         L = fnx * fny * fnz
-        return 16 * K * L + 12 * L + 4 * (fnx - 2) * (fny - 2) * (fnz - 2)
+        interior = (fnx - 2) * (fny - 2) * (fnz - 2)
+        rec = max(1, min(interior // 8, 1 << 20)) * 7 * K * 8
+        return 16 * K * L + 12 * L + 4 * interior + rec
 
 
 def _grains(n) -> int:
