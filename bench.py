@@ -193,11 +193,11 @@ def qoi_block(ctx, p) -> dict:
         return sorted(ts)[reps // 2] * 1e3
     ms_collect = med(lambda: ctx.collect_on_device(0, capi.PF_COLLECT_CLEAR_ALPHA))
     ms_volume = med(lambda: ctx.qoi_volume(int(p["num_theta"])))
-    ms_junc = med(ctx.qoi_junctions3d)
+    ms_junc = med(ctx.qoi_junction_rows)
     ms_line = med(lambda: ctx.qoi_line(0))
     ms_host = med(lambda: ctx.collect(0, capi.PF_COLLECT_CLEAR_ALPHA), 3)
     ids, _, above = ctx.qoi_volume(int(p["num_theta"]))
-    nj = len(ctx.qoi_junctions3d())
+    nj = len(ctx.qoi_junction_rows())
     field = int(p["fnx"]) * int(p["fny"]) * int(p["fnz"]) * 4
     log(f"[bench] device QoIs: collect {ms_collect:.3f} ms, volume {ms_volume:.3f} ms ({len(ids)} grains), "
         f"junctions {ms_junc:.3f} ms ({nj}), line {ms_line:.3f} ms; "

@@ -301,16 +301,26 @@ class APTContext:
         _check(load().pf_qoi_line(self._h, cur_tip, ctypes.byref(tip), _iptr(total), _iptr(extra)))
         return tip.value, total, extra
 
-    def qoi_junctions3d(self):
-        """Device QOI3D::searchJunctionsOnImage on the collected alpha: rows
-        (i, j, k, largest count, (4 ascending ids)) in scan order."""
+    def qoi_junction_rows(self) -> np.ndarray:
+        """Device QOI3D::searchJunctionsOnImage on the collected alpha: an
+        int32 array of rows (i, j, k, largest count, 4 ascending ids) in scan
+        order.  One call when the previous call's count still fits the row
+        buffer, else a second with the exact count."""
         n = ctypes.c_int()
-        _check(load().pf_qoi_junctions3d(self._h, None, 0, ctypes.byref(n)))
-        cap = n.value
-        rows = np.zeros(8 * max(cap, 1), dtype=np.int32)
-        _check(load().pf_qoi_junctions3d(self._h, _iptr(rows), cap, ctypes.byref(n)))
-        r = rows[:8 * min(cap, n.value)].reshape(-1, 8)
-        return [(int(x[0]), int(x[1]), int(x[2]), int(x[3]), tuple(int(v) for v in x[4:])) for x in r]
+        cap = getattr(self, "_junction_cap", 1024)
+        while True:
+            rows = np.zeros(8 * cap, dtype=np.int32)
+            _check(load().pf_qoi_junctions3d(self._h, _iptr(rows), cap, ctypes.byref(n)))
+            if n.value <= cap:
+                break
+            cap = n.value + n.value // 4
+        self._junction_cap = max(cap, 1024)
+        return rows[:8 * n.value].reshape(-1, 8)
+
+    def qoi_junctions3d(self):
+        """qoi_junction_rows as tuples (i, j, k, largest count, (4 ids))."""
+        return [(int(x[0]), int(x[1]), int(x[2]), int(x[3]), tuple(int(v) for v in x[4:]))
+                for x in self.qoi_junction_rows()]
 
     # --- moving domain / nucleation / diagnostics --------------------------------
     def md_enable(self, fnz_full: int):

@@ -230,8 +230,12 @@ hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t strea
 // Device-side QoIs on a collected alpha field (qoi.hip).
 hipError_t launch_qoi_volume(const int* alpha, int fnx, int fny, int fnz, int max_id, int* counts,
                              int* first, unsigned long long* above, hipStream_t stream);
-hipError_t launch_qoi_junctions(const int* alpha, int fnx, int fny, int fnz, int* rows, int cap,
-                                unsigned* count, hipStream_t stream);
+// Junction rows in scan order: count pass into blk_cnt, scan into blk_off and
+// *count, then (cap > 0) the rows; blk_cnt/blk_off hold qoi_junction_blocks words.
+int qoi_junction_blocks(int fnx, int fny, int fnz);
+hipError_t launch_qoi_junctions(const int* alpha, int fnx, int fny, int fnz, unsigned* blk_cnt,
+                                unsigned* blk_off, int* rows, int cap, unsigned* count,
+                                hipStream_t stream);
 hipError_t launch_qoi_liquid_planes(const int* alpha, int fnx, int fny, int fnz, unsigned* liq,
                                    hipStream_t stream);
 hipError_t launch_qoi_area(const int* alpha, int fnx, int fny, int fnz, int ng, int tip, int* total,

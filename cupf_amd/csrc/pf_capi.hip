@@ -966,14 +966,20 @@ int pf_qoi_junctions3d(pf_ctx* c, int* rows, int cap, int* count)
     if (rc) return rc;
     if (cap < 0 || (cap > 0 && !rows) || !count) return fail(PF_E_INVALID, "bad rows/cap/count");
     const pf_params& p = c->p;
+    const int nblk = qoi_junction_blocks(p.fnx, p.fny, p.fnz);
     int* dr = nullptr;
     unsigned* dn = nullptr;
-    if ((rc = dalloc(&dr, (size_t)8 * std::max(cap, 1))) || (rc = dalloc(&dn, 1))) {
+    unsigned* db = nullptr;
+    if ((rc = dalloc(&dr, (size_t)8 * std::max(cap, 1))) || (rc = dalloc(&dn, 1)) ||
+        (rc = dalloc(&db, (size_t)2 * nblk))) {
         (void)hipFree(dr);
+        (void)hipFree(dn);
         return rc;
     }
     unsigned n = 0;
-    hipError_t e = launch_qoi_junctions(c->alpha, p.fnx, p.fny, p.fnz, dr, cap, dn, c->stream);
+    // rows come back in the reference's scan order (k, j, i)
+    hipError_t e = launch_qoi_junctions(c->alpha, p.fnx, p.fny, p.fnz, db, db + nblk, dr, cap, dn,
+                                        c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(&n, dn, sizeof(n), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     const int m = (int)std::min<unsigned>(n, (unsigned)cap);
@@ -981,16 +987,8 @@ int pf_qoi_junctions3d(pf_ctx* c, int* rows, int cap, int* count)
         e = hipMemcpy(rows, dr, sizeof(int) * 8 * (size_t)m, hipMemcpyDeviceToHost);
     (void)hipFree(dr);
     (void)hipFree(dn);
+    (void)hipFree(db);
     if (e != hipSuccess) return fail(PF_E_HIP, std::string("qoi junctions: ") + hipGetErrorString(e));
-    // rows were appended in any order: the reference's scan order is (k, j, i)
-    std::vector<std::array<int, 8>> v((size_t)m);
-    for (int q = 0; q < m; ++q)
-        for (int u = 0; u < 8; ++u) v[q][u] = rows[8 * (size_t)q + u];
-    std::sort(v.begin(), v.end(), [](const std::array<int, 8>& x, const std::array<int, 8>& y) {
-        return std::make_tuple(x[2], x[1], x[0]) < std::make_tuple(y[2], y[1], y[0]);
-    });
-    for (int q = 0; q < m; ++q)
-        for (int u = 0; u < 8; ++u) rows[8 * (size_t)q + u] = v[q][u];
     *count = (int)n;
     return PF_OK;
 }

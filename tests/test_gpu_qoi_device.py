@@ -2,6 +2,8 @@
 restatement of 3DgrainMPI/QOI.cpp in oracle/qoi.py, on the same collected
 alpha field: a simulated state, and an uploaded block pattern with many
 4-grain junctions and a liquid region."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -57,6 +59,18 @@ def test_device_qoi_on_block_pattern():
     ref_rows = qoi.junctions3d(alpha, *dims)
     assert len(ref_rows) > 0
     assert rows == ref_rows
+    # the field spans several junction workgroups (2048 scan indices each);
+    # a row buffer shorter than the count holds the first rows in scan order
+    assert (fnx - 2) * (fny - 2) * (fnz - 2) > 4 * 2048
+    cut = max(1, len(ref_rows) // 3)
+    buf = np.full(8 * cut, -7, dtype=np.int32)
+    cnt = ctypes.c_int()
+    assert capi.load().pf_qoi_junctions3d(ctx._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                          cut, ctypes.byref(cnt)) == 0
+    assert cnt.value == len(ref_rows)
+    got = [(int(x[0]), int(x[1]), int(x[2]), int(x[3]), tuple(int(v) for v in x[4:]))
+           for x in buf.reshape(-1, 8)]
+    assert got == ref_rows[:cut]
     ids20, cnt20, above20 = ctx.qoi_volume(20)
     a = alpha.reshape(fnz, fny, fnx)[1:-1, 1:-1, 1:-1]
     assert above20 == int((a > 20).sum())
