@@ -43,6 +43,9 @@ Prints one JSON line:
              per step for all of them, hipGraph replay, autotuned chunk depth)
              against back-to-back single runs on the same GPU; both stepping
              times, sims/h, and whether every simulation ended with the same bits
+  dropin     the phase_field command (the reference's CLI surface) on the
+             same configuration in line mode, 10000 steps: wall time, its
+             evolve timer, output written
   serving    a seeded Poisson stream (4 requests/s) of the ensemble's
              distributions with 500..5000 steps each, continuously batched
              under an HBM budget: sims/h and p50/p99 latency from arrival to
@@ -218,6 +221,57 @@ def qoi_block(ctx, p) -> dict:
     }
 
 
+def dropin_block(K: int) -> dict:
+    """The drop-in `phase_field` command on the same configuration as the
+    reference would run it: a forcing folder for a 256x256x512 window of a
+    256x256x640 column (4999 grains, G 1e6 K/m, R 0.1 m/s, z0 2 um, t_end for
+    10000 steps), line mode (-l 1: moving domain every 20 steps, 10 line-QoI
+    samples, the alpha archive in the QoI file).  Reports the command's wall
+    time, its own evolve timer and the files it wrote."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    from cupf_amd import forcing
+    cli = os.path.join(ROOT, "cupf_amd", "bin", "phase_field")
+    d = tempfile.mkdtemp(prefix="cupf_dropin_")
+    try:
+        t0 = time.perf_counter()
+        forcing.write_line_case(d, 0, n=(256, 256, 512), grains=5000, G=1.0, R=0.1, z0=2.0,
+                                t_end=4.83e-5, nts=10, line_config=True, nz_full=640, top=48.0)
+        t_case = time.perf_counter() - t0
+        out = os.path.join(d, "out")
+        os.makedirs(out)
+        cmd = [cli, "line.py", "-s", "0", "-b", "110", "-f", os.path.join(d, "forcing", "case"),
+               "-o", out, "-k", str(K), "-l", "1"]
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, cwd=d, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise RuntimeError(f"phase_field failed ({r.returncode}): {r.stdout[-800:]} {r.stderr[-800:]}")
+        m = re.search(r"time for (\d+) iterations: ([0-9.]+) s", r.stdout)
+        steps, evolve_s = (int(m.group(1)), float(m.group(2))) if m else (None, None)
+        files = {f: os.path.getsize(os.path.join(out, f)) for f in sorted(os.listdir(out))}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    res = {
+        "workload": "phase_field -l 1 -k 8 on a 256x256x512 window of a 256x256x640 column, 4999 grains, "
+                    "Mt 10000 (moving domain, 10 line-QoI samples)",
+        "command": "phase_field line.py -s 0 -b 110 -f <case>/forcing/case -o <out> -k 8 -l 1",
+        "case_write_s": round(t_case, 2),
+        "wall_s": round(wall, 2),
+        "evolve_s": evolve_s,
+        "steps": steps,
+        "evolve_ms_per_step": round(evolve_s * 1e3 / steps, 4) if steps else None,
+        "setup_and_output_s": round(wall - evolve_s, 2) if evolve_s is not None else None,
+        "output_bytes": sum(files.values()),
+        "output_files": len(files),
+    }
+    log(f"[bench] drop-in phase_field: {wall:.2f} s wall, evolve {evolve_s} s for {steps} steps "
+        f"({res['evolve_ms_per_step']} ms/step), {len(files)} files, {res['output_bytes'] / 1e6:.1f} MB")
+    return res
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -331,6 +385,7 @@ def main() -> int:
     ap.add_argument("--no-qoi", action="store_true", help="skip the device QoI timings")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in phase_field run")
     ap.add_argument("--ensemble-sims", type=int, default=64)
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only run the batched-ensemble measurement and print its JSON object")
@@ -479,6 +534,13 @@ def main() -> int:
         cpu = cpu_block(ctx, case, step_after, args.cpu_seconds, threads)
 
     qoi = None if args.no_qoi else qoi_block(ctx, p)
+    dropin = None
+    if not args.no_dropin:
+        try:
+            dropin = dropin_block(args.K)
+        except (RuntimeError, OSError, ValueError) as e:  # reported in the line, not fatal to it
+            log(f"[bench] drop-in run failed: {e}")
+            dropin = {"error": str(e)[:400]}
 
     ens = None
     if not args.no_ensemble:
@@ -540,6 +602,7 @@ def main() -> int:
         "full_run": full,
         "cpu_baseline": cpu,
         "qoi": qoi,
+        "dropin": dropin,
         "ensemble": ens,
         "serving": srv,
     }
