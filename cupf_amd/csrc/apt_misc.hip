@@ -255,10 +255,11 @@ __global__ __launch_bounds__(kBlock) void ini_kernel(float* ph0, int* arg0, floa
 // every phi is rounded to a multiple of 2^-40 and summed in int64, which is
 // independent of summation order (the reference's float atomics are not).
 __global__ __launch_bounds__(kBlock) void plane_sum_kernel(const float* __restrict__ ph,
-                                                           long long* sums, int A, int L, int K)
+                                                           long long* sums, int A, int L, int K,
+                                                           int k_from)
 {
     __shared__ long long red[kBlock];
-    const int k = blockIdx.y;
+    const int k = k_from + (int)blockIdx.y;
     long long acc = 0;
     for (int s = 0; s < K; ++s)
         for (int q = blockIdx.x * kBlock + threadIdx.x; q < A; q += gridDim.x * kBlock)
@@ -557,13 +558,14 @@ hipError_t launch_ini(float* ph0, int* arg0, float* ph1, int* arg1, const float*
 }
 
 hipError_t launch_plane_sums(const float* ph, long long* sums, int fnx, int fny, int fnz, int K,
-                             hipStream_t stream)
+                             int k_from, hipStream_t stream)
 {
     const int A = fnx * fny;
-    hipError_t e = hipMemsetAsync(sums, 0, sizeof(long long) * fnz, stream);
+    hipError_t e = hipMemsetAsync(sums + k_from, 0, sizeof(long long) * (fnz - k_from), stream);
     if (e != hipSuccess) return e;
-    dim3 grid(min(cdiv(A, kBlock), 64), fnz);
-    hipLaunchKernelGGL(plane_sum_kernel, grid, dim3(kBlock), 0, stream, ph, sums, A, A * fnz, K);
+    dim3 grid(min(cdiv(A, kBlock), 64), fnz - k_from);
+    hipLaunchKernelGGL(plane_sum_kernel, grid, dim3(kBlock), 0, stream, ph, sums, A, A * fnz, K,
+                       k_from);
     return hipGetLastError();
 }
 

@@ -1092,16 +1092,17 @@ int pf_md_tip(pf_ctx* c, int* tip)
     if (!tip) return fail(PF_E_INVALID, "null tip");
     const pf_params& p = c->p;
     const int b = c->cur;
+    int t = *tip;
+    if (t < 0 || t >= p.fnz) return fail(PF_E_INVALID, "tip out of range");
+    // the scan below reads planes t .. fnz-1 only: those are the planes summed
     HIPCHK(launch_bc_fill(c->ph[b], c->arg[b], p.fnx, p.fny, p.fnz, p.K, p.bcx, p.bcy, p.bcz,
                           c->stream));
-    HIPCHK(launch_plane_sums(c->ph[b], c->sums, p.fnx, p.fny, p.fnz, p.K, c->stream));
-    HIPCHK(hipMemcpyAsync(c->h_sums.data(), c->sums, sizeof(long long) * p.fnz,
+    HIPCHK(launch_plane_sums(c->ph[b], c->sums, p.fnx, p.fny, p.fnz, p.K, t, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_sums.data() + t, c->sums + t, sizeof(long long) * (p.fnz - t),
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     // tip_mvf scan (3DgrainMPI/PhaseField.cu:391-394) on the exact plane sums.
     const double denom = (double)p.K * p.fnx * p.fny;
-    int t = *tip;
-    if (t < 0 || t >= p.fnz) return fail(PF_E_INVALID, "tip out of range");
     while (std::ldexp((double)c->h_sums[t], -40) / denom > kLS && t < p.fnz - 1) ++t;
     *tip = t;
     return PF_OK;
