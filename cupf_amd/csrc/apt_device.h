@@ -909,11 +909,12 @@ __device__ __forceinline__ void loader_direct(const PlaneLoader<K>& ld, const St
 // parts: the K id reads, then the probes.  Reads issued ahead of the probes
 // keep the LDS latency off the probe chain (a read inside the probe loop sits
 // behind the previous probe's branches and waits once per slot).
-template <int K>
+// NPT: points per slot in the LDS layout (slot stride); HX (below): row stride.
+template <int K, int NPT = kNPT>
 __device__ __forceinline__ void lds_ids(const int* __restrict__ plane, int pt, int (&ids)[K])
 {
 #pragma unroll
-    for (int p = 0; p < K; ++p) ids[p] = plane[p * kNPT + pt];
+    for (int p = 0; p < K; ++p) ids[p] = plane[p * NPT + pt];
 }
 
 template <int K, int S>
@@ -942,24 +943,25 @@ __device__ __forceinline__ void probe_ids(const int (&ids)[K], Gather<K>& g)
     }
 }
 
-template <int K, int S>
+template <int K, int S, int NPT = kNPT>
 __device__ __forceinline__ void scan_lds(const int* __restrict__ plane, int pt, Gather<K>& g)
 {
     int ids[K];
-    lds_ids<K>(plane, pt, ids);
+    lds_ids<K, NPT>(plane, pt, ids);
     probe_ids<K, S>(ids, g);
     __builtin_amdgcn_sched_barrier(0);
 }
 
 // Rolled runs of z-major positions [from, to) of one plane (ids only).
-template <int K>
+template <int K, int NPT = kNPT, int HX = kHX>
 __device__ __forceinline__ void scan_lds_run(const int* pl, int pt, int from, int to, Gather<K>& g)
 {
 #pragma unroll 1
-    for (int q = from; q < to; ++q) scan_lds<K, -1>(pl, pt + ((q / 3) % 3 - 1) * kHX + (q % 3 - 1), g);
+    for (int q = from; q < to; ++q)
+        scan_lds<K, -1, NPT>(pl, pt + ((q / 3) % 3 - 1) * HX + (q % 3 - 1), g);
 }
 
-template <int K, bool G27>
+template <int K, bool G27, int NPT = kNPT, int HX = kHX>
 __device__ __forceinline__ void scan_all_lds(const int* pD, const int* pC, const int* pU, int pt,
                                              Gather<K>& g)
 {
@@ -967,23 +969,23 @@ __device__ __forceinline__ void scan_all_lds(const int* pD, const int* pC, const
         // two id buffers: the next neighbour's reads are in flight while the
         // current one is probed
         int b0[K], b1[K];
-        lds_ids<K>(pD, pt, b0);
-        lds_ids<K>(pC, pt - kHX, b1);
+        lds_ids<K, NPT>(pD, pt, b0);
+        lds_ids<K, NPT>(pC, pt - HX, b1);
         __builtin_amdgcn_sched_barrier(0);
         probe_ids<K, 0>(b0, g);
-        lds_ids<K>(pC, pt - 1, b0);
+        lds_ids<K, NPT>(pC, pt - 1, b0);
         __builtin_amdgcn_sched_barrier(0);
         probe_ids<K, 1>(b1, g);
-        lds_ids<K>(pC, pt, b1);
+        lds_ids<K, NPT>(pC, pt, b1);
         __builtin_amdgcn_sched_barrier(0);
         probe_ids<K, 2>(b0, g);
-        lds_ids<K>(pC, pt + 1, b0);
+        lds_ids<K, NPT>(pC, pt + 1, b0);
         __builtin_amdgcn_sched_barrier(0);
         probe_ids<K, 3>(b1, g);
-        lds_ids<K>(pC, pt + kHX, b1);
+        lds_ids<K, NPT>(pC, pt + HX, b1);
         __builtin_amdgcn_sched_barrier(0);
         probe_ids<K, 4>(b0, g);
-        lds_ids<K>(pU, pt, b0);
+        lds_ids<K, NPT>(pU, pt, b0);
         __builtin_amdgcn_sched_barrier(0);
         probe_ids<K, 5>(b1, g);
         probe_ids<K, 6>(b0, g);
@@ -992,21 +994,21 @@ __device__ __forceinline__ void scan_all_lds(const int* pD, const int* pC, const
         // z-major 27-point order as in scan_all; non-stencil positions rolled,
         // each run within one plane (a run-time pick among the plane pointers
         // compiles to an indexed load from a private copy of them)
-        scan_lds_run<K>(pD, pt, 0, 4, g);
-        scan_lds<K, 0>(pD, pt, g);
-        scan_lds_run<K>(pD, pt, 5, 9, g);
-        scan_lds_run<K>(pC, pt, 9, 10, g);
-        scan_lds<K, 1>(pC, pt - kHX, g);
-        scan_lds_run<K>(pC, pt, 11, 12, g);
-        scan_lds<K, 2>(pC, pt - 1, g);
-        scan_lds<K, 3>(pC, pt, g);
-        scan_lds<K, 4>(pC, pt + 1, g);
-        scan_lds_run<K>(pC, pt, 15, 16, g);
-        scan_lds<K, 5>(pC, pt + kHX, g);
-        scan_lds_run<K>(pC, pt, 17, 18, g);
-        scan_lds_run<K>(pU, pt, 18, 22, g);
-        scan_lds<K, 6>(pU, pt, g);
-        scan_lds_run<K>(pU, pt, 23, 27, g);
+        scan_lds_run<K, NPT, HX>(pD, pt, 0, 4, g);
+        scan_lds<K, 0, NPT>(pD, pt, g);
+        scan_lds_run<K, NPT, HX>(pD, pt, 5, 9, g);
+        scan_lds_run<K, NPT, HX>(pC, pt, 9, 10, g);
+        scan_lds<K, 1, NPT>(pC, pt - HX, g);
+        scan_lds_run<K, NPT, HX>(pC, pt, 11, 12, g);
+        scan_lds<K, 2, NPT>(pC, pt - 1, g);
+        scan_lds<K, 3, NPT>(pC, pt, g);
+        scan_lds<K, 4, NPT>(pC, pt + 1, g);
+        scan_lds_run<K, NPT, HX>(pC, pt, 15, 16, g);
+        scan_lds<K, 5, NPT>(pC, pt + HX, g);
+        scan_lds_run<K, NPT, HX>(pC, pt, 17, 18, g);
+        scan_lds_run<K, NPT, HX>(pU, pt, 18, 22, g);
+        scan_lds<K, 6, NPT>(pU, pt, g);
+        scan_lds_run<K, NPT, HX>(pU, pt, 23, 27, g);
     }
 }
 
@@ -1022,13 +1024,13 @@ __device__ __forceinline__ void scan_all_lds(const int* pD, const int* pC, const
 // wrongly.  Returns false when the test fails: the fix-up kernel then applies
 // the policy in full.  (At the 256x256x512 start-up, 83-96 % of the overflowed
 // points pass.)
-template <int K>
+template <int K, int NPT = kNPT, int HX = kHX>
 __device__ __forceinline__ bool overflow_keeps_list(const int* iD, const int* iC, const int* iU,
                                                     const float* fD, const float* fC,
                                                     const float* fU, int pt, const Gather<K>& g)
 {
     auto off = [&](int s) {
-        return s == 1 ? pt - kHX : s == 2 ? pt - 1 : s == 4 ? pt + 1 : s == 5 ? pt + kHX : pt;
+        return s == 1 ? pt - HX : s == 2 ? pt - 1 : s == 4 ? pt + 1 : s == 5 ? pt + HX : pt;
     };
     float best[K];
 #pragma unroll
@@ -1045,14 +1047,14 @@ __device__ __forceinline__ bool overflow_keeps_list(const int* iD, const int* iC
             const uint32_t p = (g.pm[s] >> (4 * q)) & 15u;
             if (p == 15u) continue;
             named |= 1u << p;
-            const float v = fb[(int)p * kNPT + o];
+            const float v = fb[(int)p * NPT + o];
             best[q] = v > best[q] ? v : best[q];
         }
 #pragma unroll
         for (int p = 0; p < K; ++p) {
             if ((named >> p) & 1u) continue;
-            if (ib[p * kNPT + o] == -1) continue;
-            const float v = fb[p * kNPT + o];
+            if (ib[p * NPT + o] == -1) continue;
+            const float v = fb[p * NPT + o];
             outside = v > outside ? v : outside;
         }
     }
@@ -1088,7 +1090,7 @@ __device__ __forceinline__ unsigned fixup_append(const StepArgs& a, int ovf, uns
 // Record of an overflowed point for the fix-up: its 7 x K (id, phi) entries
 // from the LDS ring, in scan order, at record pos (structure of arrays, so the
 // wave's stores and the fix-up's loads are coalesced over consecutive entries).
-template <int K>
+template <int K, int NPT = kNPT, int HX = kHX>
 __device__ __forceinline__ void rec_store(const StepArgs& a, const int* iD, const int* iC,
                                           const int* iU, const float* fD, const float* fC,
                                           const float* fU, int pt, unsigned pos)
@@ -1097,13 +1099,13 @@ __device__ __forceinline__ void rec_store(const StepArgs& a, const int* iD, cons
     // one row at a time: only 2K values in flight, not 14K registers' worth
 #pragma unroll 1
     for (int s = 0; s < 7; ++s) {
-        const int o = s == 1 ? pt - kHX : s == 2 ? pt - 1 : s == 4 ? pt + 1 : s == 5 ? pt + kHX : pt;
+        const int o = s == 1 ? pt - HX : s == 2 ? pt - 1 : s == 4 ? pt + 1 : s == 5 ? pt + HX : pt;
         const int* ib = s == 0 ? iD : (s == 6 ? iU : iC);
         const float* fb = s == 0 ? fD : (s == 6 ? fU : fC);
 #pragma unroll
         for (int p = 0; p < K; ++p) {
-            r.id[(s * K + p) * r.cap + (int)pos] = ib[p * kNPT + o];
-            r.ph[(s * K + p) * r.cap + (int)pos] = fb[p * kNPT + o];
+            r.id[(s * K + p) * r.cap + (int)pos] = ib[p * NPT + o];
+            r.ph[(s * K + p) * r.cap + (int)pos] = fb[p * NPT + o];
         }
     }
 }
@@ -1196,6 +1198,23 @@ __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld
     }
 }
 
+// The LDS ring of a step workgroup: 3 planes of a 66x6 tile region (tile plus
+// halo), every slot's id and phi.  One allocation per K, shared by the tile
+// march and the edge strip (edge_tile), which never run at the same time in a
+// workgroup.
+template <int K>
+struct LdsRing {
+    int id[3][K * kNPT];
+    float ph[3][K * kNPT];
+};
+
+template <int K>
+__device__ __forceinline__ LdsRing<K>& lds_ring()
+{
+    __shared__ LdsRing<K> r;
+    return r;
+}
+
 // The plan kernels' unit of work: wave gw of a simulation covers z-chunk
 // group cg = gw / (waves per group) and plan_tiles consecutive full tiles.
 // Lane l reads the activity of tile slot l / plan_rows at plane kb - 1 +
@@ -1277,8 +1296,8 @@ __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, in
                                            uint64_t need, uint64_t act, float a_s, float epsv,
                                            unsigned& nres)
 {
-    __shared__ int rid[3][K * kNPT];
-    __shared__ float rph[3][K * kNPT];
+    int (*rid)[K * kNPT] = lds_ring<K>().id;
+    float (*rph)[K * kNPT] = lds_ring<K>().ph;
     __shared__ float zs[64];  // z of the chunk's planes (a chunk has at most 61)
     const int i0 = 1 + tx * kTX, j0 = 1 + ty * kTY;
     const int lx = threadIdx.x & (kTX - 1);
@@ -1326,6 +1345,31 @@ __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, in
 // point is skipped under the same rule as a tile plane: all of its stencil's
 // tile planes empty in the source and its own tile plane empty in the
 // destination.
+// Whether interior point (i, j, k) of the +x strip must be processed: some
+// tile plane of its 3x3x3 tile neighbourhood is active in the source, or its
+// own tile plane is in the destination (the tile plane rule of the activity
+// map, per point).
+__device__ __forceinline__ bool edge_need(const StepArgs& a, int i, int j, int k, int tile)
+{
+    if (!a.use_act) return true;
+    const int xl = act_tile_x(bc_map(i - 1, a.fnx, a.bcx)), xc = act_tile_x(i);
+    const int xr = act_tile_x(bc_map(i + 1, a.fnx, a.bcx));
+    const int rl = act_tile_y(bc_map(j - 1, a.fny, a.bcy)) * a.ntx, rc = act_tile_y(j) * a.ntx;
+    const int rr = act_tile_y(bc_map(j + 1, a.fny, a.bcy)) * a.ntx;
+    const unsigned e = a.ep_src;
+    // bitwise ors: all 28 reads go out together (a short-circuit chain
+    // would issue them one memory latency apart)
+    bool any = a.act_dst[k * a.ntiles + tile] >= a.ep_dst;
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz) {
+        const unsigned* w = a.act_src + bc_map(k + dz, a.fnz, a.bcz) * a.ntiles;
+        any = any | (w[xl + rl] >= e) | (w[xc + rl] >= e) | (w[xr + rl] >= e) |
+              (w[xl + rc] >= e) | (w[xc + rc] >= e) | (w[xr + rc] >= e) |
+              (w[xl + rr] >= e) | (w[xc + rr] >= e) | (w[xr + rr] >= e);
+    }
+    return any;
+}
+
 template <int K, bool G27>
 __device__ __forceinline__ void edge_point(const StepArgs& a, int p, int k0, int k1, float a_s,
                                            float epsv)
@@ -1338,31 +1382,149 @@ __device__ __forceinline__ void edge_point(const StepArgs& a, int p, int k0, int
     const int i = 1 + a.tiles_x * kTX + (c - (j - 1) * ex);
     const bool live = k < k1;
     const int tile = act_tile_x(i) + act_tile_y(j) * a.ntx;
-    bool need = live;
-    if (live && a.use_act) {
-        const int xl = act_tile_x(bc_map(i - 1, a.fnx, a.bcx)), xc = act_tile_x(i);
-        const int xr = act_tile_x(bc_map(i + 1, a.fnx, a.bcx));
-        const int rl = act_tile_y(bc_map(j - 1, a.fny, a.bcy)) * a.ntx, rc = act_tile_y(j) * a.ntx;
-        const int rr = act_tile_y(bc_map(j + 1, a.fny, a.bcy)) * a.ntx;
-        const unsigned e = a.ep_src;
-        // bitwise ors: all 28 reads go out together (a short-circuit chain
-        // would issue them one memory latency apart)
-        bool any = a.act_dst[k * a.ntiles + tile] >= a.ep_dst;
-#pragma unroll
-        for (int dz = -1; dz <= 1; ++dz) {
-            const unsigned* w = a.act_src + bc_map(k + dz, a.fnz, a.bcz) * a.ntiles;
-            any = any | (w[xl + rl] >= e) | (w[xc + rl] >= e) | (w[xr + rl] >= e) |
-                  (w[xl + rc] >= e) | (w[xc + rc] >= e) | (w[xr + rc] >= e) |
-                  (w[xl + rr] >= e) | (w[xc + rr] >= e) | (w[xr + rr] >= e);
-        }
-        need = any;
-    }
+    const bool need = live && edge_need(a, i, j, k, tile);
     int res = 0;
     if (need) {
         res = apt_point<K, G27>(a, i, j, k, a_s, epsv);
         if (res != 0) a.act_dst[k * a.ntiles + tile] = a.ep_new;
     }
     fixup_append(a, res == 2, (unsigned)(i + j * a.fnx + k * a.fnx * a.fny));
+}
+
+// The strip is one column wide (the reference sizes its grids as 64 m + 1
+// points, e.g. 257 = 4 x 64 + 1): it runs as edge tiles (7-point gather).
+__device__ __forceinline__ bool edge_strip_is_column(const StepArgs& a)
+{
+    return a.edge_cols == a.fny - 2;
+}
+
+// One edge tile: rows 1 + 64 seg .. 64 seg + 64 of the one-column strip
+// i = fnx - 2 over planes [k0, k1).  Lanes are (row, plane), 4 planes per
+// pass.  A pass stages the columns i-1, i, i+1 of its 66 rows (halo included)
+// and 6 planes, every slot's id and phi, straight from global memory into
+// the step's LDS ring with global_load_lds (K x 1188 points, exactly the
+// ring's size), then each lane runs the tile lane's gather, keep test,
+// records and slot arithmetic with row stride 3 and plane stride 198.  A
+// lane processes its point under edge_point's activity rule (the others
+// neither store nor mark nor hand over), and a pass none of whose lanes
+// does stages nothing.  Against the direct path (neighbours through L1/L2,
+// one dependent round trip per neighbour) the staging is one round trip per
+// pass.  Used by the batched step: there the strips of the 65- and
+// 129-point-wide members are a large share of the work.  In the single step
+// kernel the extra code costs the tile march more than the strip saves
+// (registers 209 -> 240 VGPRs and more scalar spills: steady state +2.6 %,
+// late +5 %), and an out-of-line call moves its arguments to scratch or
+// spills its scalar registers throughout.  Returns the overflowed points the
+// keep test settled.
+template <int K>
+__device__ __forceinline__ unsigned edge_tile(const StepArgs& a, int seg, int k0, int k1,
+                                              float a_s, float epsv)
+{
+    unsigned nres = 0u;
+    constexpr int EW = 3, EH = kEdgeRows + 2, EPL = EW * EH, ENPT = EPL * 6;
+    static_assert(ENPT == 3 * kNPT, "an edge tile fills the step's LDS ring exactly");
+    constexpr int NE = K * ENPT;  // staged entries per pass
+    constexpr int PER = (NE + kBlock - 1) / kBlock;
+    int* eid = &lds_ring<K>().id[0][0];
+    float* eph = &lds_ring<K>().ph[0][0];
+    const int A = a.fnx * a.fny;
+    const int i = a.fnx - 2;  // the strip column
+    const int lane = threadIdx.x & 63;
+    const int ly = (int)threadIdx.x % kEdgeRows, lz = (int)threadIdx.x / kEdgeRows;
+    const int j = 1 + seg * kEdgeRows + ly;
+    const bool row_on = j <= a.fny - 2;
+    const int jc = min(j, a.fny - 2);
+    const float xi = a.x[i], yj = a.y[jc];
+    const int tile = act_tile_x(i) + act_tile_y(jc) * a.ntx;
+    const int pt = lz * EPL + (ly + 1) * EW + 1;  // in plane lz + 1 through iC / fC
+    const int* iD = eid;
+    const int* iC = eid + EPL;
+    const int* iU = eid + 2 * EPL;
+    const float* fD = eph;
+    const float* fC = eph + EPL;
+    const float* fU = eph + 2 * EPL;
+    const int wbase = threadIdx.x & ~63;
+#pragma unroll 1
+    for (int kb = k0; kb < k1; kb += kBlock / kEdgeRows) {
+        const int k = kb + lz;
+        const int kc = min(k, a.fnz - 2);
+        const bool need = row_on && k < k1 && edge_need(a, i, jc, kc, tile);
+        // the barrier also orders the previous pass's LDS reads before this
+        // pass's staging
+        if (!__syncthreads_or(need ? 1 : 0)) continue;
+#pragma unroll 1
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * kBlock + (int)threadIdx.x;
+            if (e < NE) {
+                const int p = e / ENPT;
+                const int r = e - p * ENPT;
+                const int z = r / EPL;
+                const int r2 = r - z * EPL;
+                const int y = r2 / EW;
+                const int x = r2 - y * EW;
+                const int gx = bc_map(i - 1 + x, a.fnx, a.bcx);
+                const int gy = bc_map(min(seg * kEdgeRows + y, a.fny - 1), a.fny, a.bcy);
+                const int gz = bc_map(min(kb - 1 + z, a.fnz - 1), a.fnz, a.bcz);
+                const int o = gx + gy * a.fnx + gz * A + p * a.L;
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void*)(a.arg + o),
+                    (__attribute__((address_space(3))) void*)(eid + q * kBlock + wbase), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void*)(a.ph + o),
+                    (__attribute__((address_space(3))) void*)(eph + q * kBlock + wbase), 4, 0, 0);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        Gather<K> g;
+        gather_init(g);
+        scan_all_lds<K, false, ENPT, EW>(iD, iC, iU, pt, g);
+        if (!need) {
+            g.n = 0;
+            g.ovf = false;
+        }
+        {  // the keep test of tile_plane
+            const bool was = g.ovf;
+            if (__ballot(was) != 0ull) {
+                if (was && overflow_keeps_list<K, ENPT, EW>(iD, iC, iU, fD, fC, fU, pt, g))
+                    g.ovf = false;
+                nres += (unsigned)__popcll(__ballot(was && !g.ovf));
+            }
+        }
+        const int ovf = g.ovf ? 1 : 0;
+        const int C = i + jc * a.fnx + kc * A;
+        auto phi_at = [&](int s, uint32_t p) {
+            const int o = (int)p * ENPT;
+            switch (s) {
+            case 0: return fD[o + pt];
+            case 1: return fC[o + pt - EW];
+            case 2: return fC[o + pt - 1];
+            case 3: return fC[o + pt];
+            case 4: return fC[o + pt + 1];
+            case 5: return fC[o + pt + EW];
+            default: return fU[o + pt];
+            }
+        };
+        SlotPrep<K> sp;
+        if (ovf) g.n = 0;
+        slot_prep<K>(a, g, xi, yj, a.z[kc], phi_at, sp);
+        int oid[K];
+        float oph[K];
+        bool wr;
+        slot_finish<K>(a, g, sp, a_s, epsv, phi_at, oid, oph, wr);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const bool w = need && !ovf && (wr || q >= g.n);
+            int* pa = w ? a.arg_new + C + q * a.L : a.sink_i + lane;
+            float* pp = w ? a.ph_new + C + q * a.L : a.sink_f + lane;
+            *pa = oid[q];
+            *pp = oph[q];
+        }
+        if (need && (ovf || g.n > 0)) a.act_dst[kc * a.ntiles + tile] = a.ep_new;
+        const unsigned pos = fixup_append(a, ovf, (unsigned)C, true);
+        if (pos != kNoRec) rec_store<K, ENPT, EW>(a, iD, iC, iU, fD, fC, fU, pt, pos);
+    }
+    return nres;
 }
 
 __host__ __device__ inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }

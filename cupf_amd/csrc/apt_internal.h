@@ -85,7 +85,7 @@ struct StepArgs {
     unsigned* wl_light;    // chunks listed from the end of the list (word 1 of the counter set)
     int items_chunk;       // full tiles per z chunk
     int edge_cols;         // interior columns outside the full tiles
-    int edge_items;        // edge items per z chunk: ceil(kchunk * edge_cols / kEdgePoints)
+    int edge_items;        // edge items per z chunk (edge_items_per_chunk)
     int step_blocks;       // persistent workgroups of the step kernel
     int fix_blocks;        // workgroups (one wave each) of the overflow fix-up kernel
     int fnx, fny, fnz;
@@ -144,6 +144,19 @@ __host__ __device__ inline int plan_waves(int kchunk, int nzchunk, int items_chu
 // taken first, and fewer, longer ones hold workgroups the tile items need).
 constexpr int kEdgePerLane = 1;
 constexpr int kEdgePoints = 256 * kEdgePerLane;
+// Rows of one edge tile (a one-column strip, 7-point gather): 64 rows x 4
+// planes per pass of a 256-lane workgroup.
+constexpr int kEdgeRows = 64;
+
+// Edge items per z chunk for a strip of edge_cols = width x (fny - 2) points:
+// kEdgePoints points each, or (batched step, 7-point gather, one-column
+// strip) edge tiles of kEdgeRows rows.
+inline int edge_items_per_chunk(int edge_cols, int fny, int kchunk, bool gather27, bool batched)
+{
+    if (edge_cols == 0) return 0;
+    if (batched && !gather27 && edge_cols == fny - 2) return (fny - 2 + kEdgeRows - 1) / kEdgeRows;
+    return (int)(((long long)kchunk * edge_cols + kEdgePoints - 1) / kEdgePoints);
+}
 
 // Most simulations one batch may hold.
 constexpr int kMaxBatch = 1024;

@@ -387,7 +387,7 @@ int batch_upload(pf_batch* b, const std::vector<int>& first_step, int use_iuc, B
         m.a = make_step_args(c, c->cur, t, use_iuc);  // takes this step's epoch from the context
         m.a.kchunk = b->kchunk;
         m.a.nzchunk = (p.fnz - 2 + b->kchunk - 1) / b->kchunk;
-        m.a.edge_items = (int)(((long long)b->kchunk * c->edge_cols + kEdgePoints - 1) / kEdgePoints);
+        m.a.edge_items = edge_items_per_chunk(c->edge_cols, p.fny, b->kchunk, p.gather27 != 0, true);
         m.a.step_blocks = b->step_blocks;
         m.a.fix_blocks = b->fix_blocks;
         for (int q = 0; q < 2; ++q) {
@@ -538,7 +538,7 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         c->tiles_x = tiles_x;
         c->tiles_y = (p.fny - 2 + 3) / 4;
         c->edge_cols = ex * (p.fny - 2);
-        c->edge_items = (int)(((long long)c->kchunk * c->edge_cols + kEdgePoints - 1) / kEdgePoints);
+        c->edge_items = edge_items_per_chunk(c->edge_cols, p.fny, c->kchunk, p.gather27 != 0, false);
         c->items_chunk = c->tiles_x * c->tiles_y;
         c->nzchunk = (p.fnz - 2 + c->kchunk - 1) / c->kchunk;
     }

@@ -145,9 +145,14 @@ __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
             const int e = e0 - zc * a.edge_items;
             const int k0 = 1 + zc * a.kchunk;
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+            if (!G27 && edge_strip_is_column(a)) {
+                const unsigned nres = edge_tile<K>(a, e, k0, k1, a_s, epsv);
+                if ((threadIdx.x & 63) == 0 && nres) atomicAdd(a.overflow, (unsigned long long)nres);
+            } else {
 #pragma unroll 1
-            for (int r = 0; r < kEdgePerLane; ++r)
-                edge_point<K, G27>(a, e * kEdgePoints + r * kBlock + threadIdx.x, k0, k1, a_s, epsv);
+                for (int r = 0; r < kEdgePerLane; ++r)
+                    edge_point<K, G27>(a, e * kEdgePoints + r * kBlock + threadIdx.x, k0, k1, a_s, epsv);
+            }
         } else {
             const int g = b.wl_item[w];
             const int zc = g / a.items_chunk;
