@@ -1119,7 +1119,9 @@ __device__ __forceinline__ void rec_store(const StepArgs& a, const int* iD, cons
 // order, so the arithmetic only waits for the point's reads, never for the
 // prefetch, and staging only for the prefetch, never for the stores; every
 // path into an iteration has the same operations in flight.
-template <int K, bool G27>
+// REC: hand overflowed points to the fix-up with a record of their entries
+// (the step kernel variant used while neighbourhoods overflow).
+template <int K, bool G27, bool REC = true>
 __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld, int (*rid)[K * kNPT],
                                            float (*rph)[K * kNPT], const float* zs, int i, int j,
                                            int k, int k0, int tile, int pt, bool lane_on, float xi,
@@ -1189,8 +1191,8 @@ __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld
     if (ovf || g.n > 0) a.act_dst[k * a.ntiles + tile] = a.ep_new;
     // Overflowed lanes are redone after the step by the per-point fix-up, from
     // a record of their entries where one fits (7-point gather).
-    const unsigned pos = fixup_append(a, ovf, (unsigned)C, !G27);
-    if (!G27 && pos != kNoRec)
+    const unsigned pos = fixup_append(a, ovf, (unsigned)C, !G27 && REC);
+    if (!G27 && REC && pos != kNoRec)
         rec_store<K>(a, rid[sD], rid[sC], rid[sU], rph[sD], rph[sC], rph[sU], pt, pos);
     if (stage_next) {
         __syncthreads();
@@ -1291,7 +1293,7 @@ __device__ __forceinline__ void plan_wave(const StepArgs& a, int gw, int lane, b
 }
 
 // nres (wave-uniform) accumulates the overflowed points the tile resolved.
-template <int K, bool G27>
+template <int K, bool G27, bool REC = true>
 __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, int k0, int k1,
                                            uint64_t need, uint64_t act, float a_s, float epsv,
                                            unsigned& nres)
@@ -1331,7 +1333,7 @@ __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, in
                       rph[(k + 1) % 3]);
         __builtin_amdgcn_s_waitcnt(0);  // the three planes are in LDS (barrier in tile_plane)
         do {
-            tile_plane<K, G27>(a, ld, rid, rph, zs, i, j, k, k0, tile, pt, lane_on, xi, yj, a_s,
+            tile_plane<K, G27, REC>(a, ld, rid, rph, zs, i, j, k, k0, tile, pt, lane_on, xi, yj, a_s,
                                epsv, need_at(k + 1), act_at(k + 2), nres);
             ++k;
         } while (need_at(k));
@@ -1409,13 +1411,14 @@ __device__ __forceinline__ bool edge_strip_is_column(const StepArgs& a)
 // neither store nor mark nor hand over), and a pass none of whose lanes
 // does stages nothing.  Against the direct path (neighbours through L1/L2,
 // one dependent round trip per neighbour) the staging is one round trip per
-// pass.  Used by the batched step: there the strips of the 65- and
-// 129-point-wide members are a large share of the work.  In the single step
-// kernel the extra code costs the tile march more than the strip saves
-// (registers 209 -> 240 VGPRs and more scalar spills: steady state +2.6 %,
-// late +5 %), and an out-of-line call moves its arguments to scratch or
-// spills its scalar registers throughout.  Returns the overflowed points the
-// keep test settled.
+// pass, and overflowed points get the keep test and records (the direct path
+// hands every one to the fix-up without a record).  Used by the batched step
+// and by the single step kernel's overflow variant: the extra code costs the
+// tile march registers (209 -> 240 VGPRs, more scalar spills; +2.6 % in
+// steady state, +5 % late), so the variant without it runs once overflow has
+// stopped.  (An out-of-line call moved the kernel's arguments to scratch or
+// spilled its scalar registers throughout.)  Returns the overflowed points
+// the keep test settled.
 template <int K>
 __device__ __forceinline__ unsigned edge_tile(const StepArgs& a, int seg, int k0, int k1,
                                               float a_s, float epsv)

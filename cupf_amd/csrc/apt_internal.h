@@ -104,6 +104,14 @@ struct StepArgs {
     float tan_angle, cos_angle;
     int Nx, Ny, Nz;
     float X0_mac, Y0_mac, Z0_mac, Dx_mac;
+    // The host's step kernel variant (not read by the step kernel): nonzero
+    // for the one with fix-up records and edge tiles, used while
+    // neighbourhoods overflow.  The fix-up reports for that choice: it writes
+    // launch_index to seen[0] and, when its list is not empty, to seen[1]
+    // (pinned host memory; null: no report).
+    unsigned long long* seen;
+    unsigned long long launch_index;
+    int ovf_variant;
 };
 
 // One simulation of a batch, in device memory.  `a` holds the arguments of

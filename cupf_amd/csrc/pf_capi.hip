@@ -70,6 +70,12 @@ struct pf_ctx {
     unsigned long long* wl_need = nullptr;
     unsigned long long* wl_act = nullptr;
     unsigned long long nlaunch = 0;  // step launches so far (selects the counter set)
+    // Step kernel variant: the fix-up's report (pinned host memory: launch of
+    // the last fix-up, launch of the last one with a non-empty list), and the
+    // launch up to which the overflow variant is used regardless (a new state).
+    unsigned long long* seen = nullptr;
+    unsigned long long ovf_until = 0;
+    int variant_force = -1;         // CUPF_STEP_VARIANT: 0 / 1 fixes the variant (tests, timing)
     int nzchunk = 0, edge_cols = 0, edge_items = 0, items_chunk = 0;
     int tiles_x = 0, tiles_y = 0;   // LDS tiles (partial ones included)
     int step_blocks = 0;            // persistent step workgroups (occupancy x CUs)
@@ -151,6 +157,22 @@ int ctx_check(pf_ctx* c)
 
 int buf_index(pf_ctx* c, int which) { return which == 0 ? c->cur : 1 - c->cur; }
 
+// Launches without a reported overflow after which the step kernel without
+// fix-up records and edge tiles is used (7-point gather).  The report lags the
+// launches by the queue's depth; either variant gives the same bits.
+constexpr unsigned long long kOvfQuiet = 64;
+
+int ovf_variant(const pf_ctx* c)
+{
+    if (c->p.gather27) return 0;
+    if (c->variant_force >= 0) return c->variant_force;
+    if (c->nlaunch < c->ovf_until || !c->seen) return 1;
+    const volatile unsigned long long* r = c->seen;
+    const unsigned long long last = r[1], now = r[0];
+    if (now == 0) return 1;  // nothing reported yet
+    return now < last + kOvfQuiet ? 1 : 0;
+}
+
 StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
 {
     const pf_params& p = c->p;
@@ -195,7 +217,11 @@ StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
     a.nty = c->nty;
     a.ntiles = c->ntiles;
     a.edge_cols = c->edge_cols;
-    a.edge_items = c->edge_items;
+    a.ovf_variant = ovf_variant(c);
+    a.seen = c->seen;
+    a.launch_index = c->nlaunch + 1;
+    a.edge_items = edge_items_per_chunk(c->edge_cols, p.fny, c->kchunk, p.gather27 != 0,
+                                        a.ovf_variant != 0);
     a.items_chunk = c->items_chunk;
     a.step_blocks = c->step_blocks;
     a.fix_blocks = c->fix_blocks;
@@ -327,6 +353,7 @@ void free_ctx(pf_ctx* c)
                     c->wl_item, c->wl_need, c->wl_act, c->rec};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    if (c->seen) (void)hipHostFree(c->seen);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -522,6 +549,7 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         if (v >= 1 && v <= 61) c->kchunk = v;  // plane masks of a chunk fit 64 bits
     }
     if (const char* na = getenv("CUPF_NO_ACTIVITY")) c->use_act = atoi(na) ? 0 : 1;
+    if (const char* sv = getenv("CUPF_STEP_VARIANT")) c->variant_force = atoi(sv) ? 1 : 0;
     c->ntx = (p.fnx - 2 + 63) / 64;
     c->nty = (p.fny - 2 + 3) / 4;
     c->ntiles = c->ntx * c->nty;
@@ -606,6 +634,12 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         hipMemsetAsync(c->act[1], 0, (size_t)c->ntiles * p.fnz * sizeof(unsigned), c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return bail(fail(PF_E_HIP, "initial fill failed"));
+    if (hipHostMalloc(&c->seen, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        c->seen = nullptr;
+        return bail(fail(PF_E_NOMEM, "hipHostMalloc of the step variant report failed"));
+    }
+    c->seen[0] = c->seen[1] = 0ull;
+    c->ovf_until = kOvfQuiet;
     *out = c;
     return PF_OK;
 }
@@ -716,6 +750,7 @@ int pf_init_from_phi_alpha(pf_ctx* c, const float* phi0, const int* alpha0)
     HIPCHK(act_rebuild(c, 1));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_state = true;
+    c->ovf_until = c->nlaunch + kOvfQuiet;  // a new state may overflow
     return PF_OK;
 }
 
@@ -730,6 +765,7 @@ int pf_reinit_from_collect(pf_ctx* c)
     HIPCHK(act_rebuild(c, 0));
     HIPCHK(act_rebuild(c, 1));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->ovf_until = c->nlaunch + kOvfQuiet;  // a new state may overflow
     return PF_OK;
 }
 
@@ -745,6 +781,7 @@ int pf_upload_state(pf_ctx* c, int which, const float* ph, const int* arg)
     HIPCHK(act_rebuild(c, b));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_state = true;
+    c->ovf_until = c->nlaunch + kOvfQuiet;  // a new state may overflow
     return PF_OK;
 }
 

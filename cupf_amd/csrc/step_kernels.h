@@ -16,7 +16,9 @@ namespace {
 // the edge items first (direct path, slowest per point), then the listed tile
 // chunks.  Every workgroup leaves once the items are exhausted; nothing waits
 // on another workgroup.
-template <int K, bool G27>
+// OVF: the variant for steps whose neighbourhoods overflow K (fix-up records,
+// edge tiles); the host picks it per launch (StepArgs::ovf_variant).
+template <int K, bool G27, bool OVF>
 __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
 {
     __shared__ unsigned s_q;
@@ -41,9 +43,13 @@ __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
             const int e = (int)q - zc * a.edge_items;
             const int k0 = 1 + zc * a.kchunk;
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+            if (!G27 && OVF && edge_strip_is_column(a)) {
+                nres += edge_tile<K>(a, e, k0, k1, a_s, epsv);
+            } else {
 #pragma unroll 1
-            for (int r = 0; r < kEdgePerLane; ++r)
-                edge_point<K, G27>(a, e * kEdgePoints + r * kBlock + threadIdx.x, k0, k1, a_s, epsv);
+                for (int r = 0; r < kEdgePerLane; ++r)
+                    edge_point<K, G27>(a, e * kEdgePoints + r * kBlock + threadIdx.x, k0, k1, a_s, epsv);
+            }
         } else {
             // heavy chunks from the front of the list, then light ones from the back
             const unsigned w = q - ne < nh ? q - ne : cap - 1u - (q - ne - nh);
@@ -54,7 +60,7 @@ __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
             const int tx = t - ty * a.tiles_x;
             const int k0 = 1 + zc * a.kchunk;
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-            tile_march<K, G27>(a, tx, ty, k0, k1, a.wl_need[w], a.wl_act[w], a_s, epsv, nres);
+            tile_march<K, G27, OVF>(a, tx, ty, k0, k1, a.wl_need[w], a.wl_act[w], a_s, epsv, nres);
         }
     }
     if ((threadIdx.x & 63) == 0 && nres) atomicAdd(a.overflow, (unsigned long long)nres);
@@ -81,6 +87,10 @@ __global__ __launch_bounds__(kFixLanes) void apt_fixup(StepArgs a_arg)
     __syncthreads();
     const StepArgs& a = s_a;
     const unsigned n = __atomic_load_n(a.fix_count, __ATOMIC_RELAXED);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.seen) {  // for the host's variant choice
+        if (n) a.seen[1] = a.launch_index;
+        a.seen[0] = a.launch_index;
+    }
     if (n == 0u) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.overflow, (unsigned long long)n);
     const float a_s = 1.0f + 3.0f * a.kin_delta;
@@ -217,7 +227,10 @@ hipError_t launch_step_t(const StepArgs& a, hipStream_t stream, hipEvent_t ev_be
     hipError_t e = launch_plan(a, stream);
     if (e != hipSuccess) return e;
     if (ev_planned) (void)hipEventRecord(ev_planned, stream);
-    hipLaunchKernelGGL((apt_step_tiled<K, G27>), grid, block, 0, stream, a);
+    if (!G27 && a.ovf_variant)
+        hipLaunchKernelGGL((apt_step_tiled<K, G27, !G27>), grid, block, 0, stream, a);
+    else
+        hipLaunchKernelGGL((apt_step_tiled<K, G27, false>), grid, block, 0, stream, a);
     if (ev_mid) (void)hipEventRecord(ev_mid, stream);
     hipLaunchKernelGGL((apt_fixup<K, G27>), fgrid, fblock, 0, stream, a);
     if (ev_end) (void)hipEventRecord(ev_end, stream);
@@ -247,7 +260,7 @@ int slots_t(int device, int which)
         return 0;
     hipError_t e;
     switch (which) {
-    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<K, G27>, kBlock, 0); break;
+    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<K, G27, !G27>, kBlock, 0); break;
     case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_step<K, G27>, kBlock, 0); break;
     case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_fixup<K, G27>, kFixLanes, 0); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_fixup<K, G27>, kFixLanes, 0); break;

@@ -1,12 +1,14 @@
-"""Overflow fix-up (neighbourhoods holding more than K distinct grain ids,
-3DgrainMPI/APTPhaseField.cu:222-231) through the per-lane LDS tables.
+"""Overflow handling (neighbourhoods holding more than K distinct grain ids,
+3DgrainMPI/APTPhaseField.cu:222-231): the keep test in the tile, the fix-up
+records, the streaming top-K fix-up and the edge tiles of a one-column strip.
 
 The cases have the 256x256x512 benchmark's grain density (grains ~3.6 cells
 wide, so most points of the start-up transient overflow), grain ids spread over
 [1, 2^31 - 2] (the range nucleated ids take), and the fix-up grid cut to two
-workgroups, so that every lane takes dozens of overflowed points in turn and
-reuses its table for each.  Bar: ids and phi bit-exact against the oracle and
-equal overflow counts; batched members equal to their single runs.
+workgroups, so that every lane takes dozens of overflowed points in turn.  Bar:
+ids and phi bit-exact against the oracle and equal overflow counts; batched
+members equal to their single runs; both step kernel variants (with records
+and edge tiles, and without) equal to each other and to the oracle.
 """
 import numpy as np
 import pytest
@@ -72,3 +74,47 @@ def test_batched_fixup_tables_with_large_ids(monkeypatch):
     b.close()
     for q, ctx in enumerate(ctxs):
         assert ensemble.state_hash(ctx) == alone[q], f"member {q} differs from its single run"
+
+
+def test_step_variants_same_bits(monkeypatch):
+    """The step kernel with fix-up records and edge tiles, the one without,
+    and the host's choice between them (the overflow variant while the
+    fix-up reports overflow, then the other) end with the same bits, on a
+    65-point-wide grid (a one-column +x strip), through the start-up overflow
+    (150 steps, the overflow variant also against the oracle) and past it
+    (400 steps, where the host's choice has moved to the plain variant)."""
+    c = _dense_case("ovf-variants", (64, 40, 24), seed=23)
+    p = c.params
+    assert p["fnx"] - 2 == 65  # one full 64-wide tile and a one-column strip
+    ctxs = {}
+    for v in ("0", "1", "auto"):
+        if v == "auto":
+            monkeypatch.delenv("CUPF_STEP_VARIANT", raising=False)
+        else:
+            monkeypatch.setenv("CUPF_STEP_VARIANT", v)
+        ctxs[v] = gpu_context(c)
+    monkeypatch.delenv("CUPF_STEP_VARIANT", raising=False)
+
+    def state(ctx):
+        ctx.apply_bc(0)
+        ph, arg = ctx.download_state(0)
+        return ph.view(np.int32).copy(), arg.copy(), ctx.overflow_count()
+
+    sim = oracle_sim(c)
+    sim.step(150, 1)
+    for s0, n in ((1, 150), (151, 250)):
+        got = {}
+        for v, ctx in ctxs.items():
+            ctx.step(n, s0)
+            got[v] = state(ctx)
+            print(f"[variants] step {s0 + n - 1} variant {v}: overflow {got[v][2]}", flush=True)
+        if s0 == 1:
+            assert got["1"][2] == sim.overflow and sim.overflow > 0
+            bad, dphi = compare_states(ctxs["1"], sim)
+            assert bad == 0 and dphi == 0.0, first_mismatch(ctxs["1"], sim, 8, p["fnx"], p["fny"], p["fnz"])
+        for v in ("0", "auto"):
+            assert np.array_equal(got[v][1], got["1"][1]), (v, s0)
+            assert np.array_equal(got[v][0], got["1"][0]), (v, s0)
+            assert got[v][2] == got["1"][2]
+    for ctx in ctxs.values():
+        ctx.close()
