@@ -1142,7 +1142,7 @@ __device__ __forceinline__ void tile_plane(const StepArgs& a, PlaneLoader<K>& ld
     // the kept set (the wave skips this when no lane overflowed); the rest go
     // to the fix-up kernel.  nres counts the wave's settled points for the
     // overflow diagnostic.
-    if (!G27) {
+    if (!G27 && REC) {
         const bool was = g.ovf;
         if (__ballot(was) != 0ull) {
             if (was && overflow_keeps_list<K>(rid[sD], rid[sC], rid[sU], rph[sD], rph[sC], rph[sU], pt, g))
