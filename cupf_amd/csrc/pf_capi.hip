@@ -28,6 +28,12 @@ struct pf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Marks every kLeadEvery launches (kLeadSlots of them in turn): before a
+    // slot is reused, the host waits for the launch it marked, so it runs at
+    // most kLeadEvery * kLeadSlots launches ahead of the device and the fix-up's
+    // overflow report it chooses the step variant by is at most that old.
+    static constexpr int kLeadEvery = 32, kLeadSlots = 8;
+    hipEvent_t lead[kLeadSlots] = {};
     int L = 0;
     int cur = 0;                    // index of the current buffer
     float* ph[2] = {nullptr, nullptr};
@@ -161,6 +167,17 @@ int buf_index(pf_ctx* c, int which) { return which == 0 ? c->cur : 1 - c->cur; }
 // fix-up records and edge tiles is used (7-point gather).  The report lags the
 // launches by the queue's depth; either variant gives the same bits.
 constexpr unsigned long long kOvfQuiet = 64;
+
+// Before launch c->nlaunch: every kLeadEvery launches, wait for the launch
+// marked kLeadEvery * kLeadSlots launches ago, then mark this one.
+hipError_t bound_lead(pf_ctx* c)
+{
+    if (c->nlaunch % pf_ctx::kLeadEvery != 0) return hipSuccess;
+    hipEvent_t& e = c->lead[(c->nlaunch / pf_ctx::kLeadEvery) % pf_ctx::kLeadSlots];
+    hipError_t r = hipEventSynchronize(e);  // an event never recorded completes at once
+    if (r != hipSuccess) return r;
+    return hipEventRecord(e, c->stream);
+}
 
 int ovf_variant(const pf_ctx* c)
 {
@@ -356,6 +373,8 @@ void free_ctx(pf_ctx* c)
     if (c->seen) (void)hipHostFree(c->seen);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->lead)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -593,6 +612,9 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         return bail(fail(PF_E_HIP, "hipStreamCreate failed"));
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return bail(fail(PF_E_HIP, "hipEventCreate failed"));
+    for (hipEvent_t& e : c->lead)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(PF_E_HIP, "hipEventCreate failed"));
     const size_t nst = (size_t)L * p.K;
     for (int b = 0; b < 2; ++b) {
         if ((rc = dalloc(&c->ph[b], nst)) || (rc = dalloc(&c->arg[b], nst))) return bail(rc);
@@ -838,6 +860,7 @@ static int step_impl(pf_ctx* c, int nsteps, int first_step, int flags)
         // t_cur_step = (2*kt+1)*dt*tau0 in float (3DgrainMPI/APTPhaseField.cu:863,919).
         const float t = s * p.dt * p.tau0;
         HIPCHK(act_renew_if_needed(c));
+        HIPCHK(bound_lead(c));
         StepArgs a = make_step_args(c, c->cur, t, use_iuc);
         HIPCHK(launch_step(a, p.K, p.gather27, c->stream));
         ++c->nlaunch;
@@ -890,6 +913,7 @@ int pf_step_profiled(pf_ctx* c, int nsteps, int first_step, int flags, float* ms
         const int s = first_step + q;
         const float t = s * p.dt * p.tau0;
         err = act_renew_if_needed(c);
+        if (err == hipSuccess) err = bound_lead(c);
         if (err != hipSuccess) break;
         StepArgs a = make_step_args(c, c->cur, t, use_iuc);
         err = launch_step(a, p.K, p.gather27, c->stream, ev[4 * q], ev[4 * q + 1], ev[4 * q + 2],
