@@ -1413,12 +1413,8 @@ __device__ __forceinline__ bool edge_strip_is_column(const StepArgs& a)
 // one dependent round trip per neighbour) the staging is one round trip per
 // pass, and overflowed points get the keep test and records (the direct path
 // hands every one to the fix-up without a record).  Used by the batched step
-// and by the single step kernel's overflow variant: the extra code costs the
-// tile march registers (209 -> 240 VGPRs, more scalar spills; +2.6 % in
-// steady state, +5 % late), so the variant without it runs once overflow has
-// stopped.  (An out-of-line call moved the kernel's arguments to scratch or
-// spilled its scalar registers throughout.)  Returns the overflowed points
-// the keep test settled.
+// and by the single step's edge kernel of overflow steps (apt_edge).  Returns
+// the overflowed points the keep test settled.
 template <int K>
 __device__ __forceinline__ unsigned edge_tile(const StepArgs& a, int seg, int k0, int k1,
                                               float a_s, float epsv)

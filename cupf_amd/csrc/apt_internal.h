@@ -31,6 +31,12 @@ constexpr unsigned kRecFlag = 0x80000000u;
 // registers (it is at its scalar register limit).
 constexpr int kFixHeader = 64;
 
+// Fix-up list length from which a step counts as overflowing for the host's
+// choice of step kernel variant: below it the plain variant's fix-up (every
+// overflowed point, gathered again through L1/L2) costs less than the
+// overflow variant's records, keep test and edge kernel.
+constexpr unsigned kOvfHeavy = 32768u;
+
 // Leftover columns at the +x end narrower than this take the direct path;
 // wider ones form a partial LDS tile (its idle lanes cost less than the direct
 // path's per-point neighbour loads).
@@ -70,8 +76,10 @@ struct StepArgs {
     unsigned ep_new;       // epoch the destination buffer gets from this step
     int use_act;           // 0: process every plane (the map is still maintained)
     int ntx, nty, ntiles;  // activity tiles (64x4, partial ones included)
-    // Work of one step: nzchunk * edge_items edge items (kEdgePoints points
-    // of the leftover columns each), then the tile chunks listed by the plan kernel:
+    // Work of one step: nzchunk * edge_items edge items (the leftover columns:
+    // kEdgePoints points or one edge tile each; the first work indices, or a
+    // kernel of their own in a single overflow step), and the tile chunks
+    // listed by the plan kernel:
     // wl_item = z chunk * items_chunk + full tile, with the chunk's plane masks.
     // Chunks with more than half their planes to process are listed from the
     // front (wl_count[0] of them), the others from the back of the list
@@ -104,11 +112,11 @@ struct StepArgs {
     float tan_angle, cos_angle;
     int Nx, Ny, Nz;
     float X0_mac, Y0_mac, Z0_mac, Dx_mac;
-    // The host's step kernel variant (not read by the step kernel): nonzero
-    // for the one with fix-up records and edge tiles, used while
+    // The host's step kernel variant (not read by the kernels): nonzero for
+    // the one with fix-up records, keep test and the edge kernel, used while
     // neighbourhoods overflow.  The fix-up reports for that choice: it writes
-    // launch_index to seen[0] and, when its list is not empty, to seen[1]
-    // (pinned host memory; null: no report).
+    // launch_index to seen[0] and, when its list holds kOvfHeavy points or
+    // more, to seen[1] (pinned host memory; null: no report).
     unsigned long long* seen;
     unsigned long long launch_index;
     int ovf_variant;
@@ -157,12 +165,12 @@ constexpr int kEdgePoints = 256 * kEdgePerLane;
 constexpr int kEdgeRows = 64;
 
 // Edge items per z chunk for a strip of edge_cols = width x (fny - 2) points:
-// kEdgePoints points each, or (batched step, 7-point gather, one-column
-// strip) edge tiles of kEdgeRows rows.
-inline int edge_items_per_chunk(int edge_cols, int fny, int kchunk, bool gather27, bool batched)
+// kEdgePoints points each, or (tiles: batched or overflow steps, 7-point
+// gather, one-column strip) edge tiles of kEdgeRows rows.
+inline int edge_items_per_chunk(int edge_cols, int fny, int kchunk, bool gather27, bool tiles)
 {
     if (edge_cols == 0) return 0;
-    if (batched && !gather27 && edge_cols == fny - 2) return (fny - 2 + kEdgeRows - 1) / kEdgeRows;
+    if (tiles && !gather27 && edge_cols == fny - 2) return (fny - 2 + kEdgeRows - 1) / kEdgeRows;
     return (int)(((long long)kchunk * edge_cols + kEdgePoints - 1) / kEdgePoints);
 }
 
@@ -222,32 +230,41 @@ int batch_kernel_slots(int K, int gather27, int device);
 int fixup_kernel_slots(int K, int gather27, int device, bool batch);
 // Batched step: plan, persistent step, fix-up and advance kernels.
 hipError_t launch_batch_step(const BatchArgs& b, int K, int gather27, hipStream_t stream);
+// The second stream of an overflow step's edge kernel and the events that
+// order it: `ready` (recorded on the step's stream after the plan kernel),
+// `done` (on the edge stream after the edge kernel; the fix-up waits for it).
+struct EdgeLaunch {
+    hipStream_t stream;
+    hipEvent_t ready, done;
+};
+
 // Pieces shared by the per-K launchers.
 hipError_t launch_plan(const StepArgs& a, hipStream_t stream);
 hipError_t launch_batch_plan(const BatchArgs& b, hipStream_t stream);
 hipError_t launch_batch_advance(const BatchArgs& b, hipStream_t stream);
-hipError_t launch_step_k5_g7(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
-                             hipEvent_t e2, hipEvent_t e3);
+hipError_t launch_step_k5_g7(const StepArgs& a, hipStream_t stream, const EdgeLaunch* edge,
+                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3);
 hipError_t launch_batch_k5_g7(const BatchArgs& b, hipStream_t stream);
 int step_slots_k5_g7(int device, int which);
-hipError_t launch_step_k5_g27(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
-                             hipEvent_t e2, hipEvent_t e3);
+hipError_t launch_step_k5_g27(const StepArgs& a, hipStream_t stream, const EdgeLaunch* edge,
+                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3);
 hipError_t launch_batch_k5_g27(const BatchArgs& b, hipStream_t stream);
 int step_slots_k5_g27(int device, int which);
-hipError_t launch_step_k8_g7(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
-                             hipEvent_t e2, hipEvent_t e3);
+hipError_t launch_step_k8_g7(const StepArgs& a, hipStream_t stream, const EdgeLaunch* edge,
+                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3);
 hipError_t launch_batch_k8_g7(const BatchArgs& b, hipStream_t stream);
 int step_slots_k8_g7(int device, int which);
-hipError_t launch_step_k8_g27(const StepArgs& a, hipStream_t stream, hipEvent_t e0, hipEvent_t e1,
-                             hipEvent_t e2, hipEvent_t e3);
+hipError_t launch_step_k8_g27(const StepArgs& a, hipStream_t stream, const EdgeLaunch* edge,
+                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3);
 hipError_t launch_batch_k8_g27(const BatchArgs& b, hipStream_t stream);
 int step_slots_k8_g27(int device, int which);
 // Work-list plan, main step kernel, then the overflow fix-up kernels.
 // Optional events are recorded before the plan, after it, after the main
 // kernel and after the fix-ups.
 hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,
-                       hipEvent_t ev_begin = nullptr, hipEvent_t ev_planned = nullptr,
-                       hipEvent_t ev_mid = nullptr, hipEvent_t ev_end = nullptr);
+                       const EdgeLaunch* edge, hipEvent_t ev_begin = nullptr,
+                       hipEvent_t ev_planned = nullptr, hipEvent_t ev_mid = nullptr,
+                       hipEvent_t ev_end = nullptr);
 // Device-side QoIs on a collected alpha field (qoi.hip).
 hipError_t launch_qoi_volume(const int* alpha, int fnx, int fny, int fnz, int max_id, int* counts,
                              int* first, unsigned long long* above, hipStream_t stream);

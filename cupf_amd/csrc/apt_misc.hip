@@ -57,7 +57,7 @@ __device__ __forceinline__ bool plan_heavy(uint64_t need, int kchunk)
 // group, tiles), see plan_wave.  A tile chunk is listed only if some plane of
 // it needs processing, with its plane masks.  Edge items are not listed: the
 // first nzchunk * edge_items work indices are edge items, always taken (their
-// points decide one by one).
+// points decide one by one), or the edge kernel's in overflow steps.
 __global__ __launch_bounds__(kBlock) void apt_plan(StepArgs a)
 {
     constexpr int kPlanWaves = kBlock / 64;
@@ -481,13 +481,15 @@ hipError_t launch_batch_advance(const BatchArgs& b, hipStream_t stream)
 }
 
 hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,
-                       hipEvent_t ev_begin, hipEvent_t ev_planned, hipEvent_t ev_mid,
-                       hipEvent_t ev_end)
+                       const EdgeLaunch* edge, hipEvent_t ev_begin, hipEvent_t ev_planned,
+                       hipEvent_t ev_mid, hipEvent_t ev_end)
 {
-    if (K == 5) return gather27 ? launch_step_k5_g27(a, stream, ev_begin, ev_planned, ev_mid, ev_end)
-                                : launch_step_k5_g7(a, stream, ev_begin, ev_planned, ev_mid, ev_end);
-    if (K == 8) return gather27 ? launch_step_k8_g27(a, stream, ev_begin, ev_planned, ev_mid, ev_end)
-                                : launch_step_k8_g7(a, stream, ev_begin, ev_planned, ev_mid, ev_end);
+    if (K == 5)
+        return gather27 ? launch_step_k5_g27(a, stream, edge, ev_begin, ev_planned, ev_mid, ev_end)
+                        : launch_step_k5_g7(a, stream, edge, ev_begin, ev_planned, ev_mid, ev_end);
+    if (K == 8)
+        return gather27 ? launch_step_k8_g27(a, stream, edge, ev_begin, ev_planned, ev_mid, ev_end)
+                        : launch_step_k8_g7(a, stream, edge, ev_begin, ev_planned, ev_mid, ev_end);
     return hipErrorInvalidValue;
 }
 

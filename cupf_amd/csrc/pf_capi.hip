@@ -28,11 +28,14 @@ struct pf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // the stream of the overflow steps' edge kernel and the events that fork
+    // it off `stream` and join it back before the fix-up
+    EdgeLaunch edge = {nullptr, nullptr, nullptr};
     // Marks every kLeadEvery launches (kLeadSlots of them in turn): before a
     // slot is reused, the host waits for the launch it marked, so it runs at
     // most kLeadEvery * kLeadSlots launches ahead of the device and the fix-up's
     // overflow report it chooses the step variant by is at most that old.
-    static constexpr int kLeadEvery = 32, kLeadSlots = 8;
+    static constexpr int kLeadEvery = 8, kLeadSlots = 4;
     hipEvent_t lead[kLeadSlots] = {};
     int L = 0;
     int cur = 0;                    // index of the current buffer
@@ -163,10 +166,18 @@ int ctx_check(pf_ctx* c)
 
 int buf_index(pf_ctx* c, int which) { return which == 0 ? c->cur : 1 - c->cur; }
 
-// Launches without a reported overflow after which the step kernel without
-// fix-up records and edge tiles is used (7-point gather).  The report lags the
-// launches by the queue's depth; either variant gives the same bits.
-constexpr unsigned long long kOvfQuiet = 64;
+// Launches without a reported heavy overflow (a fix-up list of kOvfHeavy
+// points or more) after which the step kernel without fix-up records and keep
+// test, its edge items inside it, is used (7-point gather).  The report lags
+// the launches by the queue's depth (at most kLeadEvery * kLeadSlots); either
+// variant gives the same bits.  The lag and the quiet period are kept short:
+// the overflow variant's edge kernel costs 15-35 us per launch once little
+// overflows.
+constexpr unsigned long long kOvfQuiet = 16;
+// Launches after a new state that take the overflow variant whatever the
+// reports say (those in flight are still the old state's).
+constexpr unsigned long long kOvfNew =
+    (unsigned long long)(pf_ctx::kLeadEvery * pf_ctx::kLeadSlots) + kOvfQuiet;
 
 // Before launch c->nlaunch: every kLeadEvery launches, wait for the launch
 // marked kLeadEvery * kLeadSlots launches ago, then mark this one.
@@ -375,6 +386,9 @@ void free_ctx(pf_ctx* c)
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->lead)
         if (e) (void)hipEventDestroy(e);
+    if (c->edge.ready) (void)hipEventDestroy(c->edge.ready);
+    if (c->edge.done) (void)hipEventDestroy(c->edge.done);
+    if (c->edge.stream) (void)hipStreamDestroy(c->edge.stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -615,6 +629,11 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
     for (hipEvent_t& e : c->lead)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
             return bail(fail(PF_E_HIP, "hipEventCreate failed"));
+    if (c->edge_items > 0 && !p.gather27 &&
+        (hipStreamCreateWithFlags(&c->edge.stream, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&c->edge.ready, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&c->edge.done, hipEventDisableTiming) != hipSuccess))
+        return bail(fail(PF_E_HIP, "edge stream or event creation failed"));
     const size_t nst = (size_t)L * p.K;
     for (int b = 0; b < 2; ++b) {
         if ((rc = dalloc(&c->ph[b], nst)) || (rc = dalloc(&c->arg[b], nst))) return bail(rc);
@@ -661,7 +680,7 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         return bail(fail(PF_E_NOMEM, "hipHostMalloc of the step variant report failed"));
     }
     c->seen[0] = c->seen[1] = 0ull;
-    c->ovf_until = kOvfQuiet;
+    c->ovf_until = kOvfNew;
     *out = c;
     return PF_OK;
 }
@@ -772,7 +791,7 @@ int pf_init_from_phi_alpha(pf_ctx* c, const float* phi0, const int* alpha0)
     HIPCHK(act_rebuild(c, 1));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_state = true;
-    c->ovf_until = c->nlaunch + kOvfQuiet;  // a new state may overflow
+    c->ovf_until = c->nlaunch + kOvfNew;  // a new state may overflow
     return PF_OK;
 }
 
@@ -787,7 +806,7 @@ int pf_reinit_from_collect(pf_ctx* c)
     HIPCHK(act_rebuild(c, 0));
     HIPCHK(act_rebuild(c, 1));
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->ovf_until = c->nlaunch + kOvfQuiet;  // a new state may overflow
+    c->ovf_until = c->nlaunch + kOvfNew;  // a new state may overflow
     return PF_OK;
 }
 
@@ -803,7 +822,7 @@ int pf_upload_state(pf_ctx* c, int which, const float* ph, const int* arg)
     HIPCHK(act_rebuild(c, b));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_state = true;
-    c->ovf_until = c->nlaunch + kOvfQuiet;  // a new state may overflow
+    c->ovf_until = c->nlaunch + kOvfNew;  // a new state may overflow
     return PF_OK;
 }
 
@@ -862,7 +881,7 @@ static int step_impl(pf_ctx* c, int nsteps, int first_step, int flags)
         HIPCHK(act_renew_if_needed(c));
         HIPCHK(bound_lead(c));
         StepArgs a = make_step_args(c, c->cur, t, use_iuc);
-        HIPCHK(launch_step(a, p.K, p.gather27, c->stream));
+        HIPCHK(launch_step(a, p.K, p.gather27, c->stream, &c->edge));
         ++c->nlaunch;
         c->ep[1 - c->cur] = a.ep_new;
         c->cur = 1 - c->cur;
@@ -916,8 +935,8 @@ int pf_step_profiled(pf_ctx* c, int nsteps, int first_step, int flags, float* ms
         if (err == hipSuccess) err = bound_lead(c);
         if (err != hipSuccess) break;
         StepArgs a = make_step_args(c, c->cur, t, use_iuc);
-        err = launch_step(a, p.K, p.gather27, c->stream, ev[4 * q], ev[4 * q + 1], ev[4 * q + 2],
-                          ev[4 * q + 3]);
+        err = launch_step(a, p.K, p.gather27, c->stream, &c->edge, ev[4 * q], ev[4 * q + 1],
+                          ev[4 * q + 2], ev[4 * q + 3]);
         if (err != hipSuccess) break;
         ++c->nlaunch;
         c->ep[1 - c->cur] = a.ep_new;

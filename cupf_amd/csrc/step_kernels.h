@@ -16,15 +16,16 @@ namespace {
 // the edge items first (direct path, slowest per point), then the listed tile
 // chunks.  Every workgroup leaves once the items are exhausted; nothing waits
 // on another workgroup.
-// OVF: the variant for steps whose neighbourhoods overflow K (fix-up records,
-// edge tiles); the host picks it per launch (StepArgs::ovf_variant).
+// OVF: the variant for steps whose neighbourhoods overflow K (in-tile keep
+// test, fix-up records; 7-point gather only); the host picks it per launch
+// (StepArgs::ovf_variant).  Its edge items are apt_edge's, on a second stream.
 template <int K, bool G27, bool OVF>
 __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
 {
     __shared__ unsigned s_q;
     const float a_s = 1.0f + 3.0f * a.kin_delta;
     const float epsv = -4.0f * a.kin_delta / a_s;
-    const unsigned ne = (unsigned)(a.nzchunk * a.edge_items);
+    const unsigned ne = OVF ? 0u : (unsigned)(a.nzchunk * a.edge_items);
     const unsigned nh = __atomic_load_n(a.wl_count, __ATOMIC_RELAXED);
     const unsigned n = ne + nh + __atomic_load_n(a.wl_light, __ATOMIC_RELAXED);
     const unsigned cap = (unsigned)(a.nzchunk * a.items_chunk);
@@ -43,13 +44,9 @@ __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
             const int e = (int)q - zc * a.edge_items;
             const int k0 = 1 + zc * a.kchunk;
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
-            if (!G27 && OVF && edge_strip_is_column(a)) {
-                nres += edge_tile<K>(a, e, k0, k1, a_s, epsv);
-            } else {
 #pragma unroll 1
-                for (int r = 0; r < kEdgePerLane; ++r)
-                    edge_point<K, G27>(a, e * kEdgePoints + r * kBlock + threadIdx.x, k0, k1, a_s, epsv);
-            }
+            for (int r = 0; r < kEdgePerLane; ++r)
+                edge_point<K, G27>(a, e * kEdgePoints + r * kBlock + threadIdx.x, k0, k1, a_s, epsv);
         } else {
             // heavy chunks from the front of the list, then light ones from the back
             const unsigned w = q - ne < nh ? q - ne : cap - 1u - (q - ne - nh);
@@ -64,6 +61,36 @@ __global__ __launch_bounds__(kBlock) void apt_step_tiled(StepArgs a)
         }
     }
     if ((threadIdx.x & 63) == 0 && nres) atomicAdd(a.overflow, (unsigned long long)nres);
+}
+
+// The edge items (the +x strip outside the tiles) of an overflow step (7-point
+// gather): one workgroup per item; a one-column strip as edge tiles (keep
+// test and records), other strips on the direct path.  Out of the step kernel
+// so its tile march keeps registers (206 against 240 VGPRs, fewer scalar
+// spills), and on a second stream that forks after the plan kernel and joins
+// before the fix-up, so its workgroups run beside the step kernel's.  The two
+// write disjoint points and activity words and meet only in the fix-up
+// list's atomics.  (Steps without overflow keep their edge items in the step
+// kernel: there the second launch and its idle workgroups, each holding its
+// LDS, cost more than they save.)
+template <int K>
+__global__ __launch_bounds__(kBlock) void apt_edge(StepArgs a)
+{
+    const float a_s = 1.0f + 3.0f * a.kin_delta;
+    const float epsv = -4.0f * a.kin_delta / a_s;
+    const int q = blockIdx.x;
+    const int zc = q / a.edge_items;
+    const int e = q - zc * a.edge_items;
+    const int k0 = 1 + zc * a.kchunk;
+    const int k1 = min(k0 + a.kchunk, a.fnz - 1);
+    if (edge_strip_is_column(a)) {
+        const unsigned nres = edge_tile<K>(a, e, k0, k1, a_s, epsv);
+        if ((threadIdx.x & 63) == 0 && nres) atomicAdd(a.overflow, (unsigned long long)nres);
+    } else {
+#pragma unroll 1
+        for (int r = 0; r < kEdgePerLane; ++r)
+            edge_point<K, false>(a, e * kEdgePoints + r * kBlock + threadIdx.x, k0, k1, a_s, epsv);
+    }
 }
 
 // Fix-up pass for the points of the last apt_step launch whose neighbourhood
@@ -88,7 +115,7 @@ __global__ __launch_bounds__(kFixLanes) void apt_fixup(StepArgs a_arg)
     const StepArgs& a = s_a;
     const unsigned n = __atomic_load_n(a.fix_count, __ATOMIC_RELAXED);
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.seen) {  // for the host's variant choice
-        if (n) a.seen[1] = a.launch_index;
+        if (n >= kOvfHeavy) a.seen[1] = a.launch_index;
         a.seen[0] = a.launch_index;
     }
     if (n == 0u) return;
@@ -219,18 +246,30 @@ __global__ __launch_bounds__(kFixLanes) void batch_fixup(BatchArgs b)
 }
 
 template <int K, bool G27>
-hipError_t launch_step_t(const StepArgs& a, hipStream_t stream, hipEvent_t ev_begin,
-                         hipEvent_t ev_planned, hipEvent_t ev_mid, hipEvent_t ev_end)
+hipError_t launch_step_t(const StepArgs& a, hipStream_t stream, const EdgeLaunch* edge,
+                         hipEvent_t ev_begin, hipEvent_t ev_planned, hipEvent_t ev_mid,
+                         hipEvent_t ev_end)
 {
     dim3 grid(a.step_blocks), block(kBlock), fgrid(a.fix_blocks), fblock(kFixLanes);
+    // overflow steps (7-point gather) run their edge items as apt_edge
+    const int ne = (!G27 && a.ovf_variant) ? a.nzchunk * a.edge_items : 0;
+    if (ne > 0 && !edge) return hipErrorInvalidValue;
     if (ev_begin) (void)hipEventRecord(ev_begin, stream);
     hipError_t e = launch_plan(a, stream);
     if (e != hipSuccess) return e;
     if (ev_planned) (void)hipEventRecord(ev_planned, stream);
+    if (ne > 0) {
+        if ((e = hipEventRecord(edge->ready, stream)) != hipSuccess ||
+            (e = hipStreamWaitEvent(edge->stream, edge->ready, 0)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL((apt_edge<K>), dim3(ne), block, 0, edge->stream, a);
+        if ((e = hipEventRecord(edge->done, edge->stream)) != hipSuccess) return e;
+    }
     if (!G27 && a.ovf_variant)
         hipLaunchKernelGGL((apt_step_tiled<K, G27, !G27>), grid, block, 0, stream, a);
     else
         hipLaunchKernelGGL((apt_step_tiled<K, G27, false>), grid, block, 0, stream, a);
+    if (ne > 0 && (e = hipStreamWaitEvent(stream, edge->done, 0)) != hipSuccess) return e;
     if (ev_mid) (void)hipEventRecord(ev_mid, stream);
     hipLaunchKernelGGL((apt_fixup<K, G27>), fgrid, fblock, 0, stream, a);
     if (ev_end) (void)hipEventRecord(ev_end, stream);
