@@ -11,8 +11,10 @@ Prints one JSON line:
              domain counts, as every one is updated)
   roofline   per-kernel durations of the same window (the run is replayed
              from the initial state with HIP events around every launch on
-             the context's stream): work-list plan, step kernel, overflow
-             fix-up.  The step kernel skips z planes of 64x4 column tiles
+             the context's stream): work-list plan, step kernel (in steps
+             with heavy overflow together with the edge kernel that runs
+             beside it on a second stream and joins before the fix-up),
+             overflow fix-up.  The step kernel skips z planes of 64x4 column tiles
              whose stencil reaches only empty slots and whose destination is
              empty (their result is known without reading anything), so the
              algorithmic bytes of a step are those of the points it processes:
@@ -584,7 +586,7 @@ def main() -> int:
             "traffic": traffic,
             "traffic_by_kernel": traffic_kernels,
             "kernel": dominant,
-            "kernels": "apt_plan + apt_step_tiled + apt_fixup (one step)",
+            "kernels": "apt_plan + apt_step_tiled (+ apt_edge beside it) + apt_fixup (one step)",
             "step_ms": round(t_all, 4),
             "plan_ms": round(avg_plan, 4),
             "kernel_ms": round(avg_main, 4),

@@ -7,8 +7,9 @@ wide, so most points of the start-up transient overflow), grain ids spread over
 [1, 2^31 - 2] (the range nucleated ids take), and the fix-up grid cut to two
 workgroups, so that every lane takes dozens of overflowed points in turn.  Bar:
 ids and phi bit-exact against the oracle and equal overflow counts; batched
-members equal to their single runs; both step kernel variants (with records
-and edge tiles, and without) equal to each other and to the oracle.
+members equal to their single runs; both step kernel variants (with records,
+keep test and the edge kernel beside it, and without) equal to each other and
+to the oracle.
 """
 import numpy as np
 import pytest
