@@ -1414,8 +1414,9 @@ __device__ __forceinline__ bool edge_strip_is_column(const StepArgs& a)
 // pass, and overflowed points get the keep test and records (the direct path
 // hands every one to the fix-up without a record).  Used by the batched step
 // and by the single step's edge kernel of overflow steps (apt_edge).  Returns
-// the overflowed points the keep test settled.
-template <int K>
+// the overflowed points the keep test settled.  REC false: neither keep test
+// nor records (every overflowed point goes to the fix-up whole).
+template <int K, bool REC = true>
 __device__ __forceinline__ unsigned edge_tile(const StepArgs& a, int seg, int k0, int k1,
                                               float a_s, float epsv)
 {
@@ -1482,7 +1483,7 @@ __device__ __forceinline__ unsigned edge_tile(const StepArgs& a, int seg, int k0
             g.n = 0;
             g.ovf = false;
         }
-        {  // the keep test of tile_plane
+        if (REC) {  // the keep test of tile_plane
             const bool was = g.ovf;
             if (__ballot(was) != 0ull) {
                 if (was && overflow_keeps_list<K, ENPT, EW>(iD, iC, iU, fD, fC, fU, pt, g))
@@ -1520,8 +1521,8 @@ __device__ __forceinline__ unsigned edge_tile(const StepArgs& a, int seg, int k0
             *pp = oph[q];
         }
         if (need && (ovf || g.n > 0)) a.act_dst[kc * a.ntiles + tile] = a.ep_new;
-        const unsigned pos = fixup_append(a, ovf, (unsigned)C, true);
-        if (pos != kNoRec) rec_store<K, ENPT, EW>(a, iD, iC, iU, fD, fC, fU, pt, pos);
+        const unsigned pos = fixup_append(a, ovf, (unsigned)C, REC);
+        if (REC && pos != kNoRec) rec_store<K, ENPT, EW>(a, iD, iC, iU, fD, fC, fU, pt, pos);
     }
     return nres;
 }

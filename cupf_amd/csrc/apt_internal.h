@@ -180,7 +180,8 @@ constexpr int kMaxBatch = 1024;
 // A batched step over n simulations sharing K and the gather variant.  The
 // work list holds tile chunks of every simulation; cnt holds two 4-word
 // counter sets ([0] chunks listed from the front, [1] next work index, [2]
-// chunks listed from the back) used by alternate steps, selected by cnt[8].
+// chunks listed from the back) used by alternate steps, selected by cnt[8];
+// cnt[9] counts the batched fix-up's launches.
 struct BatchArgs {
     BatchSim* sims;
     int n;
@@ -194,6 +195,13 @@ struct BatchArgs {
     int wl_cap;            // list capacity: light chunks are listed down from wl_cap - 1
     int step_blocks;
     int fix_blocks;        // workgroups (one wave each) of the batched fix-up kernel
+    // The host's batch_step variant: nonzero for the one with fix-up records
+    // and keep test (7-point gather).  The batched fix-up reports for that
+    // choice (pinned host memory; null: no report): it counts its launches in
+    // cnt[9] and writes the count to seen[0] and, when the lists of all
+    // members hold kOvfHeavy points or more, to seen[1].
+    unsigned long long* seen;
+    int ovf_variant;
 };
 
 // Nucleation launch block (3DgrainMPI/APTPhaseField.cu:459-536).

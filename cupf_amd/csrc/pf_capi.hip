@@ -408,14 +408,27 @@ struct pf_batch {
     int* wl_item = nullptr;
     unsigned long long* wl_need = nullptr;
     unsigned long long* wl_act = nullptr;
-    unsigned* cnt = nullptr;        // 2 x 4 list counters + parity word
+    unsigned* cnt = nullptr;        // 2 x 4 list counters, parity word, fix-up count
     size_t wl_cap = 0;
-    // hipGraph of kGraphSteps batched steps, captured for one kchunk
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_kchunk = 0;
+    // hipGraphs of kGraphSteps batched steps, one per step kernel variant,
+    // each captured for one kchunk
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int graph_kchunk[2] = {0, 0};
     static constexpr int kGraphSteps = 8;
     static constexpr int kMinChunk = 2;
+    // Step kernel variant choice, as for a single context: batch_fixup
+    // reports (pinned) the index within the current run of its step and of
+    // the last step whose lists held kOvfHeavy points or more in all.
+    unsigned long long* seen = nullptr;
+    unsigned long long nstep = 0;     // batched steps enqueued over the batch's life
+    unsigned long long seen_base = 0; // nstep when the current run began
+    unsigned long long heavy_at = 0;  // last step known to overflow heavily (1-based)
+    int variant_force = -1;           // CUPF_STEP_VARIANT
+    // lead bound: a mark every kGraphSteps steps, kLeadSlots of them in turn
+    static constexpr int kLeadSlots = 4;
+    hipEvent_t lead[kLeadSlots] = {};
+    unsigned long long lead_n = 0, next_mark = 0;
 };
 
 namespace {
@@ -423,11 +436,16 @@ namespace {
 void batch_free(pf_batch* b)
 {
     (void)hipSetDevice(b->device);
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (int v = 0; v < 2; ++v) {
+        if (b->gexec[v]) (void)hipGraphExecDestroy(b->gexec[v]);
+        if (b->graph[v]) (void)hipGraphDestroy(b->graph[v]);
+    }
     void* bufs[] = {b->sims, b->wl_sim, b->wl_item, b->wl_need, b->wl_act, b->cnt};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
+    if (b->seen) (void)hipHostFree(b->seen);
+    for (hipEvent_t e : b->lead)
+        if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -468,7 +486,7 @@ int batch_upload(pf_batch* b, const std::vector<int>& first_step, int use_iuc, B
         edge_total += m.a.nzchunk * m.a.edge_items;
     }
     HIPCHK(hipMemcpyAsync(b->sims, h.data(), sizeof(BatchSim) * n, hipMemcpyHostToDevice, b->stream));
-    HIPCHK(hipMemsetAsync(b->cnt, 0, 9 * sizeof(unsigned), b->stream));
+    HIPCHK(hipMemsetAsync(b->cnt, 0, 10 * sizeof(unsigned), b->stream));
     ba.sims = b->sims;
     ba.n = n;
     ba.plan_total = plan_total;
@@ -481,7 +499,35 @@ int batch_upload(pf_batch* b, const std::vector<int>& first_step, int use_iuc, B
     ba.cnt = b->cnt;
     ba.step_blocks = b->step_blocks;
     ba.fix_blocks = b->fix_blocks;
+    ba.seen = b->seen;
+    ba.ovf_variant = 0;
     return PF_OK;
+}
+
+// The step kernel variant of the next batched step (1: fix-up records and
+// keep test), from the fix-up's reports as for a single context.
+int batch_variant(pf_batch* b)
+{
+    if (b->g27) return 0;
+    if (b->variant_force >= 0) return b->variant_force;
+    const volatile unsigned long long* r = b->seen;
+    const unsigned long long now = r[0], last = r[1];
+    if (last) b->heavy_at = std::max(b->heavy_at, b->seen_base + last);
+    return b->seen_base + now < b->heavy_at + kOvfQuiet ? 1 : 0;
+}
+
+// Before the steps from b->nstep on: every kGraphSteps steps, wait for the
+// mark set kLeadSlots marks before, then mark the stream, so the host runs at
+// most ~kLeadSlots * kGraphSteps steps ahead of the reports it chooses by.
+hipError_t batch_bound_lead(pf_batch* b)
+{
+    if (b->nstep < b->next_mark) return hipSuccess;
+    hipEvent_t& e = b->lead[b->lead_n % pf_batch::kLeadSlots];
+    hipError_t r = hipEventSynchronize(e);  // an event never recorded completes at once
+    if (r != hipSuccess) return r;
+    ++b->lead_n;
+    b->next_mark = b->nstep + pf_batch::kGraphSteps;
+    return hipEventRecord(e, b->stream);
 }
 
 // The host side of nsteps batched steps (what step_impl does per launch),
@@ -510,34 +556,46 @@ int batch_run(pf_batch* b, int nsteps, std::vector<int>& first_step, int use_iuc
     BatchArgs ba;
     int rc = batch_upload(b, first_step, use_iuc, ba);
     if (rc) return rc;
+    // the previous run ended with a synchronisation: fold its last reports in,
+    // then count this run's steps from 1 (batch_upload cleared the device count)
+    (void)batch_variant(b);
+    b->seen[0] = b->seen[1] = 0ull;
+    b->seen_base = b->nstep;
     int left = nsteps;
-    if (use_graph && nsteps >= pf_batch::kGraphSteps) {
-        if (!b->gexec || b->graph_kchunk != b->kchunk) {
-            if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-            if (b->graph) (void)hipGraphDestroy(b->graph);
-            b->gexec = nullptr;
-            b->graph = nullptr;
-            HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t e = hipSuccess;
-            for (int q = 0; q < pf_batch::kGraphSteps && e == hipSuccess; ++q)
-                e = launch_batch_step(ba, b->K, b->g27, b->stream);
-            hipGraph_t g = nullptr;
-            const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-            if (e != hipSuccess || e2 != hipSuccess) {
-                if (g) (void)hipGraphDestroy(g);
-                return fail(PF_E_HIP, std::string("graph capture: ") +
-                                          hipGetErrorString(e != hipSuccess ? e : e2));
+    while (left > 0) {
+        HIPCHK(batch_bound_lead(b));
+        const int v = batch_variant(b);
+        ba.ovf_variant = v;
+        if (use_graph && left >= pf_batch::kGraphSteps) {
+            if (!b->gexec[v] || b->graph_kchunk[v] != b->kchunk) {
+                if (b->gexec[v]) (void)hipGraphExecDestroy(b->gexec[v]);
+                if (b->graph[v]) (void)hipGraphDestroy(b->graph[v]);
+                b->gexec[v] = nullptr;
+                b->graph[v] = nullptr;
+                HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                hipError_t e = hipSuccess;
+                for (int q = 0; q < pf_batch::kGraphSteps && e == hipSuccess; ++q)
+                    e = launch_batch_step(ba, b->K, b->g27, b->stream);
+                hipGraph_t g = nullptr;
+                const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+                if (e != hipSuccess || e2 != hipSuccess) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return fail(PF_E_HIP, std::string("graph capture: ") +
+                                              hipGetErrorString(e != hipSuccess ? e : e2));
+                }
+                b->graph[v] = g;
+                HIPCHK(hipGraphInstantiate(&b->gexec[v], b->graph[v], nullptr, nullptr, 0));
+                b->graph_kchunk[v] = b->kchunk;
             }
-            b->graph = g;
-            HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-            b->graph_kchunk = b->kchunk;
-        }
-        while (left >= pf_batch::kGraphSteps) {
-            HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+            HIPCHK(hipGraphLaunch(b->gexec[v], b->stream));
             left -= pf_batch::kGraphSteps;
+            b->nstep += pf_batch::kGraphSteps;
+        } else {
+            HIPCHK(launch_batch_step(ba, b->K, b->g27, b->stream));
+            --left;
+            ++b->nstep;
         }
     }
-    for (; left > 0; --left) HIPCHK(launch_batch_step(ba, b->K, b->g27, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     batch_mirror(b, nsteps);
     for (int& s : first_step) s += nsteps;
@@ -1400,8 +1458,17 @@ int pf_batch_create(pf_ctx* const* ctxs, int n, pf_batch** out)
     int rc;
     if ((rc = dalloc(&b->sims, (size_t)n)) || (rc = dalloc(&b->wl_sim, b->wl_cap + 1)) ||
         (rc = dalloc(&b->wl_item, b->wl_cap + 1)) || (rc = dalloc(&b->wl_need, b->wl_cap + 1)) ||
-        (rc = dalloc(&b->wl_act, b->wl_cap + 1)) || (rc = dalloc(&b->cnt, 9)))
+        (rc = dalloc(&b->wl_act, b->wl_cap + 1)) || (rc = dalloc(&b->cnt, 10)))
         return bail(rc);
+    for (hipEvent_t& e : b->lead)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(PF_E_HIP, "hipEventCreate failed"));
+    if (hipHostMalloc(&b->seen, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        b->seen = nullptr;
+        return bail(fail(PF_E_NOMEM, "hipHostMalloc of the step variant report failed"));
+    }
+    b->seen[0] = b->seen[1] = 0ull;
+    if (const char* sv = getenv("CUPF_STEP_VARIANT")) b->variant_force = atoi(sv) ? 1 : 0;
     *out = b;
     return PF_OK;
 }

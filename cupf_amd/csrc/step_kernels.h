@@ -149,7 +149,7 @@ __device__ __forceinline__ int batch_find(const BatchSim* sims, int n, int q, bo
 // Batched APTrhs_psi: the persistent grid takes the edge items of every
 // simulation, then the tile chunks the batched plan kernel listed, each with
 // its own simulation's step arguments.
-template <int K, bool G27>
+template <int K, bool G27, bool OVF>
 __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
 {
     __shared__ unsigned s_q;
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
             const int k0 = 1 + zc * a.kchunk;
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
             if (!G27 && edge_strip_is_column(a)) {
-                const unsigned nres = edge_tile<K>(a, e, k0, k1, a_s, epsv);
+                const unsigned nres = edge_tile<K, OVF>(a, e, k0, k1, a_s, epsv);
                 if ((threadIdx.x & 63) == 0 && nres) atomicAdd(a.overflow, (unsigned long long)nres);
             } else {
 #pragma unroll 1
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void batch_step(BatchArgs b)
             const int k0 = 1 + zc * a.kchunk;
             const int k1 = min(k0 + a.kchunk, a.fnz - 1);
             unsigned nres = 0u;
-            tile_march<K, G27>(a, tx, ty, k0, k1, b.wl_need[w], b.wl_act[w], a_s, epsv, nres);
+            tile_march<K, G27, OVF>(a, tx, ty, k0, k1, b.wl_need[w], b.wl_act[w], a_s, epsv, nres);
             if ((threadIdx.x & 63) == 0 && nres) atomicAdd(a.overflow, (unsigned long long)nres);
         }
         __syncthreads();  // s_q is rewritten for the next item
@@ -226,6 +226,11 @@ __global__ __launch_bounds__(kFixLanes) void batch_fixup(BatchArgs b)
     }
     __syncthreads();
     const unsigned total = s_pre[b.n];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && b.seen) {  // for the host's variant choice
+        const unsigned long long idx = ++b.cnt[9];
+        if (total >= kOvfHeavy) b.seen[1] = idx;
+        b.seen[0] = idx;
+    }
     for (unsigned g = blockIdx.x * kFixLanes + threadIdx.x; g < total; g += gridDim.x * kFixLanes) {
         int lo = 0, hi = b.n - 1;  // last simulation whose range starts at or before g
         while (lo < hi) {
@@ -282,7 +287,10 @@ hipError_t launch_batch_t(const BatchArgs& b, hipStream_t stream)
     dim3 grid(b.step_blocks), block(kBlock);
     hipError_t e = launch_batch_plan(b, stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((batch_step<K, G27>), grid, block, 0, stream, b);
+    if (!G27 && b.ovf_variant)
+        hipLaunchKernelGGL((batch_step<K, G27, !G27>), grid, block, 0, stream, b);
+    else
+        hipLaunchKernelGGL((batch_step<K, G27, false>), grid, block, 0, stream, b);
     hipLaunchKernelGGL((batch_fixup<K, G27>), dim3(b.fix_blocks), dim3(kFixLanes), 0, stream, b);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -300,7 +308,7 @@ int slots_t(int device, int which)
     hipError_t e;
     switch (which) {
     case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_step_tiled<K, G27, !G27>, kBlock, 0); break;
-    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_step<K, G27>, kBlock, 0); break;
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_step<K, G27, !G27>, kBlock, 0); break;
     case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, apt_fixup<K, G27>, kFixLanes, 0); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_fixup<K, G27>, kFixLanes, 0); break;
     }

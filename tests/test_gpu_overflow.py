@@ -119,3 +119,36 @@ def test_step_variants_same_bits(monkeypatch):
             assert got[v][2] == got["1"][2]
     for ctx in ctxs.values():
         ctx.close()
+
+
+def test_batch_variants_same_bits(monkeypatch):
+    """batch_step with fix-up records and keep test, the one without, and the
+    host's choice between them (per 8-step graph replay, then per step) give
+    every member the bits of its single run, through the start-up overflow
+    and past it; one member has a one-column +x strip (edge tiles with and
+    without records)."""
+    members = [_dense_case("bv0", (64, 40, 24), seed=29), _dense_case("bv1", (40, 40, 20), seed=31)]
+    assert members[0].params["fnx"] - 2 == 65
+    alone = []
+    for c in members:
+        ctx = gpu_context(c)
+        ctx.step(200, 1)
+        alone.append((ensemble.state_hash(ctx), ctx.overflow_count()))
+        assert ctx.overflow_count() > 0
+        ctx.close()
+    for v in ("0", "1", "auto"):
+        if v == "auto":
+            monkeypatch.delenv("CUPF_STEP_VARIANT", raising=False)
+        else:
+            monkeypatch.setenv("CUPF_STEP_VARIANT", v)
+        ctxs = [gpu_context(c) for c in members]
+        b = capi.Batch(ctxs)
+        b.step(96, capi.PF_BATCH_GRAPH)
+        b.step(104)
+        b.close()
+        print(f"[batch variants] variant {v}: overflow {[x.overflow_count() for x in ctxs]}", flush=True)
+        for q, ctx in enumerate(ctxs):
+            assert ensemble.state_hash(ctx) == alone[q][0], f"variant {v}: member {q} differs from its single run"
+            assert ctx.overflow_count() == alone[q][1]
+            ctx.close()
+    monkeypatch.delenv("CUPF_STEP_VARIANT", raising=False)
