@@ -595,6 +595,38 @@ __device__ __forceinline__ void topk_point(const int* __restrict__ arg, const fl
     __builtin_amdgcn_sched_barrier(0);
 }
 
+// The K (id, phi) entries of one neighbour point, loaded one neighbour ahead
+// of their top-K pushes so that the fix-up's chain of neighbour loads
+// overlaps the pushes instead of taking one memory round trip per neighbour.
+template <int K>
+struct NbrEntries {
+    int id[K];
+    float ph[K];
+};
+
+template <int K>
+__device__ __forceinline__ void entries_load(const int* __restrict__ arg, const float* __restrict__ ph,
+                                             int base, int L, NbrEntries<K>& e)
+{
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        e.id[p] = arg[base + p * L];
+        e.ph[p] = ph[base + p * L];
+    }
+}
+
+// topk_point's pushes for entries loaded earlier; P: the point's position in
+// the scan order.
+template <int K>
+__device__ __forceinline__ void entries_push(const NbrEntries<K>& e, int P, TopK<K>& c)
+{
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        if (e.id[p] == -1) continue;
+        topk_push(c, e.id[p], e.ph[p], P * K + p);
+    }
+}
+
 // Pass A restricted to the kept ids.
 template <int K, int S>
 __device__ __forceinline__ void scan_point_kept(const int* __restrict__ arg, int base, int L,
@@ -648,13 +680,17 @@ __device__ __forceinline__ void gather_overflow(const StepArgs& a, const Nbr& nb
     }
     gather_init(g);
     if (!G27) {
-        topk_point<K>(a.arg, a.ph, nbase(a, nb, 1, 1, 0), a.L, 0, c);
-        topk_point<K>(a.arg, a.ph, nbase(a, nb, 1, 0, 1), a.L, 1, c);
-        topk_point<K>(a.arg, a.ph, nbase(a, nb, 0, 1, 1), a.L, 2, c);
-        topk_point<K>(a.arg, a.ph, nbase(a, nb, 1, 1, 1), a.L, 3, c);
-        topk_point<K>(a.arg, a.ph, nbase(a, nb, 2, 1, 1), a.L, 4, c);
-        topk_point<K>(a.arg, a.ph, nbase(a, nb, 1, 2, 1), a.L, 5, c);
-        topk_point<K>(a.arg, a.ph, nbase(a, nb, 1, 1, 2), a.L, 6, c);
+        const int nbs[7] = {nbase(a, nb, 1, 1, 0), nbase(a, nb, 1, 0, 1), nbase(a, nb, 0, 1, 1),
+                            nbase(a, nb, 1, 1, 1), nbase(a, nb, 2, 1, 1), nbase(a, nb, 1, 2, 1),
+                            nbase(a, nb, 1, 1, 2)};
+        NbrEntries<K> e[2];
+        entries_load<K>(a.arg, a.ph, nbs[0], a.L, e[0]);
+#pragma unroll
+        for (int u = 0; u < 7; ++u) {
+            if (u + 1 < 7) entries_load<K>(a.arg, a.ph, nbs[u + 1], a.L, e[(u + 1) & 1]);
+            entries_push<K>(e[u & 1], u, c);
+            __builtin_amdgcn_sched_barrier(0);
+        }
         scan_point_kept<K, 0>(a.arg, nbase(a, nb, 1, 1, 0), a.L, g, c);
         scan_point_kept<K, 1>(a.arg, nbase(a, nb, 1, 0, 1), a.L, g, c);
         scan_point_kept<K, 2>(a.arg, nbase(a, nb, 0, 1, 1), a.L, g, c);
@@ -716,8 +752,14 @@ __device__ __forceinline__ void gather_overflow_rec(const int* rid, const float*
         c.pos[u] = 0;
     }
     gather_init(g);
+    NbrEntries<K> cur, nxt;
+    entries_load<K>(rid, rph, q, cap, cur);
 #pragma unroll 1
-    for (int s = 0; s < 7; ++s) topk_point<K>(rid + s * K * cap, rph + s * K * cap, q, cap, s, c);
+    for (int s = 0; s < 7; ++s) {
+        if (s + 1 < 7) entries_load<K>(rid + (s + 1) * K * cap, rph + (s + 1) * K * cap, q, cap, nxt);
+        entries_push<K>(cur, s, c);
+        cur = nxt;
+    }
     scan_point_kept<K, 0>(rid + 0 * K * cap, q, cap, g, c);
     scan_point_kept<K, 1>(rid + 1 * K * cap, q, cap, g, c);
     scan_point_kept<K, 2>(rid + 2 * K * cap, q, cap, g, c);

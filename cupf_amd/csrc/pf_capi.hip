@@ -374,6 +374,8 @@ hipError_t act_renew_if_needed(pf_ctx* c, unsigned ahead = 1u)
 void free_ctx(pf_ctx* c)
 {
     (void)hipSetDevice(c->device);
+    // a step that failed between its fork and join leaves the edge stream unjoined
+    if (c->edge.stream) (void)hipStreamSynchronize(c->edge.stream);
     void* bufs[] = {c->ph[0], c->ph[1], c->arg[0], c->arg[1], c->x, c->y, c->z, c->zbuf,
                     c->R12, c->psi_mac, c->T3d, c->phi0, c->phi, c->alpha, c->counters,
                     c->sums, c->alpha_full, c->loss, c->nstatus, c->ball,
