@@ -102,9 +102,10 @@ __global__ __launch_bounds__(kFixLanes) void apt_fixup(StepArgs a_arg)
 {
     // The arguments are read from an LDS copy: with them in scalar registers
     // the compiler keeps 229 VGPRs live (2 waves/SIMD); from LDS it needs 92
-    // for K=8 (5 waves/SIMD), and each entry is a long chain of dependent
-    // loads that only more waves in flight can overlap (more workgroups per CU
-    // made the start-up transient of the 256x256x512 run 14 % faster than 2).
+    // for K=8 (5 waves/SIMD; 120 and 4 with the neighbour entries loaded one
+    // neighbour ahead), and each entry is a long chain of dependent loads that
+    // only more waves in flight can overlap (more workgroups per CU made the
+    // start-up transient of the 256x256x512 run 14 % faster than 2).
     __shared__ StepArgs s_a;
     constexpr int kWords = (int)(sizeof(StepArgs) / 4);
     static_assert(sizeof(StepArgs) % 4 == 0 && kWords <= kFixLanes, "StepArgs staging");
