@@ -627,14 +627,10 @@ __device__ __forceinline__ void entries_push(const NbrEntries<K>& e, int P, TopK
     }
 }
 
-// Pass A restricted to the kept ids.
+// Pass A restricted to the kept ids, on one neighbour's K ids loaded earlier.
 template <int K, int S>
-__device__ __forceinline__ void scan_point_kept(const int* __restrict__ arg, int base, int L,
-                                                Gather<K>& g, const TopK<K>& kept)
+__device__ __forceinline__ void ids_kept(const int (&ids)[K], Gather<K>& g, const TopK<K>& kept)
 {
-    int ids[K];  // the K loads first, as in scan_point
-#pragma unroll
-    for (int p = 0; p < K; ++p) ids[p] = arg[base + p * L];
 #pragma unroll
     for (int p = 0; p < K; ++p) {
         const int id = ids[p];
@@ -654,6 +650,54 @@ __device__ __forceinline__ void scan_point_kept(const int* __restrict__ arg, int
             g.pm[S] = (g.pm[S] & ~(15u << sh)) | ((uint32_t)p << sh);
         }
     }
+}
+
+template <int K>
+__device__ __forceinline__ void ids_load(const int* __restrict__ arg, int base, int L, int (&ids)[K])
+{
+#pragma unroll
+    for (int p = 0; p < K; ++p) ids[p] = arg[base + p * L];
+}
+
+// Pass A restricted to the kept ids.
+template <int K, int S>
+__device__ __forceinline__ void scan_point_kept(const int* __restrict__ arg, int base, int L,
+                                                Gather<K>& g, const TopK<K>& kept)
+{
+    int ids[K];  // the K loads first, as in scan_point
+    ids_load<K>(arg, base, L, ids);
+    ids_kept<K, S>(ids, g, kept);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// Pass A restricted to the kept ids over the 7 stencil points D,B,L,C,R,T,U
+// (point u at arg[bases[u] + p * L]), each point's ids loaded while the
+// previous point's are scanned.
+template <int K>
+__device__ __forceinline__ void kept_pass7(const int* __restrict__ arg, const int (&bases)[7], int L,
+                                           Gather<K>& g, const TopK<K>& c)
+{
+    int x[K], y[K];
+    ids_load<K>(arg, bases[0], L, x);
+    ids_load<K>(arg, bases[1], L, y);
+    ids_kept<K, 0>(x, g, c);
+    __builtin_amdgcn_sched_barrier(0);
+    ids_load<K>(arg, bases[2], L, x);
+    ids_kept<K, 1>(y, g, c);
+    __builtin_amdgcn_sched_barrier(0);
+    ids_load<K>(arg, bases[3], L, y);
+    ids_kept<K, 2>(x, g, c);
+    __builtin_amdgcn_sched_barrier(0);
+    ids_load<K>(arg, bases[4], L, x);
+    ids_kept<K, 3>(y, g, c);
+    __builtin_amdgcn_sched_barrier(0);
+    ids_load<K>(arg, bases[5], L, y);
+    ids_kept<K, 4>(x, g, c);
+    __builtin_amdgcn_sched_barrier(0);
+    ids_load<K>(arg, bases[6], L, x);
+    ids_kept<K, 5>(y, g, c);
+    __builtin_amdgcn_sched_barrier(0);
+    ids_kept<K, 6>(x, g, c);
     __builtin_amdgcn_sched_barrier(0);
 }
 
@@ -691,13 +735,7 @@ __device__ __forceinline__ void gather_overflow(const StepArgs& a, const Nbr& nb
             entries_push<K>(e[u & 1], u, c);
             __builtin_amdgcn_sched_barrier(0);
         }
-        scan_point_kept<K, 0>(a.arg, nbase(a, nb, 1, 1, 0), a.L, g, c);
-        scan_point_kept<K, 1>(a.arg, nbase(a, nb, 1, 0, 1), a.L, g, c);
-        scan_point_kept<K, 2>(a.arg, nbase(a, nb, 0, 1, 1), a.L, g, c);
-        scan_point_kept<K, 3>(a.arg, nbase(a, nb, 1, 1, 1), a.L, g, c);
-        scan_point_kept<K, 4>(a.arg, nbase(a, nb, 2, 1, 1), a.L, g, c);
-        scan_point_kept<K, 5>(a.arg, nbase(a, nb, 1, 2, 1), a.L, g, c);
-        scan_point_kept<K, 6>(a.arg, nbase(a, nb, 1, 1, 2), a.L, g, c);
+        kept_pass7<K>(a.arg, nbs, a.L, g, c);
     } else {
 #pragma unroll 1
         for (int q = 0; q < 27; ++q) topk_point<K>(a.arg, a.ph, nbase_q(a, nb, q), a.L, q, c);
@@ -760,13 +798,9 @@ __device__ __forceinline__ void gather_overflow_rec(const int* rid, const float*
         entries_push<K>(cur, s, c);
         cur = nxt;
     }
-    scan_point_kept<K, 0>(rid + 0 * K * cap, q, cap, g, c);
-    scan_point_kept<K, 1>(rid + 1 * K * cap, q, cap, g, c);
-    scan_point_kept<K, 2>(rid + 2 * K * cap, q, cap, g, c);
-    scan_point_kept<K, 3>(rid + 3 * K * cap, q, cap, g, c);
-    scan_point_kept<K, 4>(rid + 4 * K * cap, q, cap, g, c);
-    scan_point_kept<K, 5>(rid + 5 * K * cap, q, cap, g, c);
-    scan_point_kept<K, 6>(rid + 6 * K * cap, q, cap, g, c);
+    const int bases[7] = {q, q + K * cap, q + 2 * K * cap, q + 3 * K * cap, q + 4 * K * cap,
+                          q + 5 * K * cap, q + 6 * K * cap};
+    kept_pass7<K>(rid, bases, cap, g, c);
 }
 
 // One interior point: pass A, then pass B.  Returns 0 if no neighbour holds a
