@@ -292,9 +292,9 @@ hipError_t launch_collect(const float* ph, const int* arg, float* phi, int* alph
                           int clear_alpha, hipStream_t stream);
 hipError_t launch_ini(float* ph0, int* arg0, float* ph1, int* arg1, const float* phi0,
                       const int* alpha0, int L, int K, hipStream_t stream);
-// Exact sums of planes k_from .. fnz-1 into sums[k_from ..] (the rest untouched).
+// Exact sums of planes k_from .. k_to-1 into sums[k_from .. k_to) (the rest untouched).
 hipError_t launch_plane_sums(const float* ph, long long* sums, int fnx, int fny, int fnz, int K,
-                             int k_from, hipStream_t stream);
+                             int k_from, int k_to, hipStream_t stream);
 hipError_t launch_move_frame(float* ph_old, int* arg_old, float* ph_new, int* arg_new, float* z,
                              float* zbuf, const int* alpha, int* alpha_full, int* loss_area,
                              int move_count, int fnx, int fny, int fnz, int K, int num_theta,

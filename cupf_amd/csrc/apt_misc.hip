@@ -560,12 +560,13 @@ hipError_t launch_ini(float* ph0, int* arg0, float* ph1, int* arg1, const float*
 }
 
 hipError_t launch_plane_sums(const float* ph, long long* sums, int fnx, int fny, int fnz, int K,
-                             int k_from, hipStream_t stream)
+                             int k_from, int k_to, hipStream_t stream)
 {
     const int A = fnx * fny;
-    hipError_t e = hipMemsetAsync(sums + k_from, 0, sizeof(long long) * (fnz - k_from), stream);
+    if (k_from < 0 || k_to > fnz || k_to <= k_from) return hipErrorInvalidValue;
+    hipError_t e = hipMemsetAsync(sums + k_from, 0, sizeof(long long) * (k_to - k_from), stream);
     if (e != hipSuccess) return e;
-    dim3 grid(min(cdiv(A, kBlock), 64), fnz - k_from);
+    dim3 grid(min(cdiv(A, kBlock), 64), k_to - k_from);
     hipLaunchKernelGGL(plane_sum_kernel, grid, dim3(kBlock), 0, stream, ph, sums, A, A * fnz, K,
                        k_from);
     return hipGetLastError();

@@ -1234,16 +1234,27 @@ int pf_md_tip(pf_ctx* c, int* tip)
     const int b = c->cur;
     int t = *tip;
     if (t < 0 || t >= p.fnz) return fail(PF_E_INVALID, "tip out of range");
-    // the scan below reads planes t .. fnz-1 only: those are the planes summed
     HIPCHK(launch_bc_fill(c->ph[b], c->arg[b], p.fnx, p.fny, p.fnz, p.K, p.bcx, p.bcy, p.bcz,
                           c->stream));
-    HIPCHK(launch_plane_sums(c->ph[b], c->sums, p.fnx, p.fny, p.fnz, p.K, t, c->stream));
-    HIPCHK(hipMemcpyAsync(c->h_sums.data() + t, c->sums + t, sizeof(long long) * (p.fnz - t),
-                          hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
     // tip_mvf scan (3DgrainMPI/PhaseField.cu:391-394) on the exact plane sums.
+    // It reads planes from t up to the first liquid one, a few planes above
+    // the tip: the sums come kTipWindow planes at a time, as the scan reaches
+    // them (the same conditions in the same order as over all planes).
+    constexpr int kTipWindow = 16;
     const double denom = (double)p.K * p.fnx * p.fny;
-    while (std::ldexp((double)c->h_sums[t], -40) / denom > kLS && t < p.fnz - 1) ++t;
+    int hi = t;  // sums known for planes [*tip, hi)
+    for (;;) {
+        if (t >= hi) {
+            const int to = std::min(hi + kTipWindow, p.fnz);
+            HIPCHK(launch_plane_sums(c->ph[b], c->sums, p.fnx, p.fny, p.fnz, p.K, hi, to, c->stream));
+            HIPCHK(hipMemcpyAsync(c->h_sums.data() + hi, c->sums + hi, sizeof(long long) * (to - hi),
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            hi = to;
+        }
+        if (!(std::ldexp((double)c->h_sums[t], -40) / denom > kLS && t < p.fnz - 1)) break;
+        ++t;
+    }
     *tip = t;
     return PF_OK;
 }

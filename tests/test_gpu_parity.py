@@ -180,6 +180,22 @@ def test_moving_domain_shift():
     assert compare_states(ctx, sim) == (0, 0.0)
 
 
+@pytest.mark.parametrize("z0,start", [(30, 1), (30, 3), (30, 29), (8, 40), (8, 47), (8, 48)])
+def test_tip_scan_windows(z0, start):
+    """The device tip scan sums planes 16 at a time as it reaches them: a tip
+    more than one window above the start, one inside the first window, a
+    start above the interface and at the top plane give the oracle's tip."""
+    c = cases.make_line_case("tipw", n=(12, 12, 46), grains=4, K=5, seed=37, z0_cells=z0,
+                             line_config=True, nz_full=96)
+    ctx, sim = _run_pair(c, 4)
+    tip_g = ctx.md_tip(start)
+    tip_o = sim.md_tip(start)
+    print(f"[tip] z0 {z0} start {start}: tip {tip_o}", flush=True)
+    assert tip_g == tip_o
+    if start == 1 and z0 == 30:
+        assert tip_o > 17  # crossed into the second window
+
+
 def test_nucleation_initial_undercooling():
     """Liquid start with uniform cooling: add_nucl after odd steps with the
     counter-based draw.  Decisions match unless a draw lands within an ulp of
