@@ -77,14 +77,16 @@ def test_batched_fixup_tables_with_large_ids(monkeypatch):
         assert ensemble.state_hash(ctx) == alone[q], f"member {q} differs from its single run"
 
 
-def test_step_variants_same_bits(monkeypatch):
-    """The step kernel with fix-up records and edge tiles, the one without,
-    and the host's choice between them (the overflow variant while the
-    fix-up reports overflow, then the other) end with the same bits, on a
+@pytest.mark.parametrize("K", [8, 5])
+def test_step_variants_same_bits(K, monkeypatch):
+    """The step kernel with fix-up records and keep test (its edge strip as
+    the edge kernel beside it), the plain one, and the host's choice between
+    them (the overflow variant after a new state and while the fix-up reports
+    heavy overflow, then the plain one) end with the same bits, on a
     65-point-wide grid (a one-column +x strip), through the start-up overflow
     (150 steps, the overflow variant also against the oracle) and past it
-    (400 steps, where the host's choice has moved to the plain variant)."""
-    c = _dense_case("ovf-variants", (64, 40, 24), seed=23)
+    (400 steps), for K=8 and K=5."""
+    c = _dense_case("ovf-variants", (64, 40, 24), seed=23, K=K)
     p = c.params
     assert p["fnx"] - 2 == 65  # one full 64-wide tile and a one-column strip
     ctxs = {}
@@ -108,11 +110,11 @@ def test_step_variants_same_bits(monkeypatch):
         for v, ctx in ctxs.items():
             ctx.step(n, s0)
             got[v] = state(ctx)
-            print(f"[variants] step {s0 + n - 1} variant {v}: overflow {got[v][2]}", flush=True)
+            print(f"[variants K={K}] step {s0 + n - 1} variant {v}: overflow {got[v][2]}", flush=True)
         if s0 == 1:
             assert got["1"][2] == sim.overflow and sim.overflow > 0
             bad, dphi = compare_states(ctxs["1"], sim)
-            assert bad == 0 and dphi == 0.0, first_mismatch(ctxs["1"], sim, 8, p["fnx"], p["fny"], p["fnz"])
+            assert bad == 0 and dphi == 0.0, first_mismatch(ctxs["1"], sim, K, p["fnx"], p["fny"], p["fnz"])
         for v in ("0", "auto"):
             assert np.array_equal(got[v][1], got["1"][1]), (v, s0)
             assert np.array_equal(got[v][0], got["1"][0]), (v, s0)
@@ -121,13 +123,14 @@ def test_step_variants_same_bits(monkeypatch):
         ctx.close()
 
 
-def test_batch_variants_same_bits(monkeypatch):
+@pytest.mark.parametrize("K", [8, 5])
+def test_batch_variants_same_bits(K, monkeypatch):
     """batch_step with fix-up records and keep test, the one without, and the
     host's choice between them (per 8-step graph replay, then per step) give
     every member the bits of its single run, through the start-up overflow
     and past it; one member has a one-column +x strip (edge tiles with and
     without records)."""
-    members = [_dense_case("bv0", (64, 40, 24), seed=29), _dense_case("bv1", (40, 40, 20), seed=31)]
+    members = [_dense_case("bv0", (64, 40, 24), seed=29, K=K), _dense_case("bv1", (40, 40, 20), seed=31, K=K)]
     assert members[0].params["fnx"] - 2 == 65
     alone = []
     for c in members:
@@ -146,7 +149,7 @@ def test_batch_variants_same_bits(monkeypatch):
         b.step(96, capi.PF_BATCH_GRAPH)
         b.step(104)
         b.close()
-        print(f"[batch variants] variant {v}: overflow {[x.overflow_count() for x in ctxs]}", flush=True)
+        print(f"[batch variants K={K}] variant {v}: overflow {[x.overflow_count() for x in ctxs]}", flush=True)
         for q, ctx in enumerate(ctxs):
             assert ensemble.state_hash(ctx) == alone[q][0], f"variant {v}: member {q} differs from its single run"
             assert ctx.overflow_count() == alone[q][1]
