@@ -1368,6 +1368,74 @@ __device__ __forceinline__ void plan_wave(const StepArgs& a, int gw, int lane, b
     }
 }
 
+// List position of this lane's chunk if `put`: a heavy chunk (more than half
+// of its planes to process) from the front of the list, a light one from the
+// back (cap - 1 downward).  Every thread of the workgroup (kPlanWaves waves)
+// calls it; waves take consecutive runs of positions in wave order.
+template <int kPlanWaves>
+__device__ __forceinline__ unsigned plan_slot(bool put, bool heavy, unsigned* front, unsigned* back,
+                                              unsigned cap, unsigned* s_cnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const unsigned long long mh = __ballot(put && heavy);
+    const unsigned long long ml = __ballot(put && !heavy);
+    if (lane == 0) {
+        s_cnt[w] = (unsigned)__popcll(mh);
+        s_cnt[kPlanWaves + 1 + w] = (unsigned)__popcll(ml);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned th = 0u, tl = 0u;
+        for (int u = 0; u < kPlanWaves; ++u) {
+            const unsigned ch = s_cnt[u], cl = s_cnt[kPlanWaves + 1 + u];
+            s_cnt[u] = th;
+            s_cnt[kPlanWaves + 1 + u] = tl;
+            th += ch;
+            tl += cl;
+        }
+        s_cnt[kPlanWaves] = th ? atomicAdd(front, th) : 0u;
+        s_cnt[2 * kPlanWaves + 1] = tl ? atomicAdd(back, tl) : 0u;
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (heavy) return s_cnt[kPlanWaves] + s_cnt[w] + (unsigned)__popcll(mh & below);
+    return cap - 1u - (s_cnt[2 * kPlanWaves + 1] + s_cnt[kPlanWaves + 1 + w] +
+                       (unsigned)__popcll(ml & below));
+}
+
+__device__ __forceinline__ bool plan_heavy(uint64_t need, int kchunk)
+{
+    return 2 * __popcll(need) > kchunk;
+}
+
+// One workgroup of a step's plan (apt_plan, and the plan half of
+// apt_fixplan): plan block pb covers waves 4*pb .. 4*pb+3; block 0 also
+// clears the counter set of the step after this one.
+__device__ __forceinline__ void plan_block(const StepArgs& a, int pb)
+{
+    constexpr int kPlanWaves = kBlock / 64;
+    __shared__ unsigned s_cnt[2 * kPlanWaves + 2];
+    const int g = pb * kBlock + (int)threadIdx.x;
+    // the counter set of the step after this one (last used two steps
+    // before, whose fix-up has finished) starts from zero
+    if (g < 4) a.cnt_next[g] = 0u;
+    const int gw = g >> 6;  // wave-uniform
+    const int lane = threadIdx.x & 63;
+    bool put = false;
+    int item = 0;
+    uint64_t need = 0ull, act = 0ull;
+    if (gw < plan_waves(a.kchunk, a.nzchunk, a.items_chunk))
+        plan_wave(a, gw, lane, put, item, need, act);
+    const unsigned q = plan_slot<kPlanWaves>(put, plan_heavy(need, a.kchunk), a.wl_count, a.wl_light,
+                                             (unsigned)(a.nzchunk * a.items_chunk), s_cnt);
+    if (put) {
+        a.wl_item[q] = item;
+        a.wl_need[q] = need;
+        a.wl_act[q] = act;
+    }
+}
+
 // nres (wave-uniform) accumulates the overflowed points the tile resolved.
 template <int K, bool G27, bool REC = true>
 __device__ __forceinline__ void tile_march(const StepArgs& a, int tx, int ty, int k0, int k1,

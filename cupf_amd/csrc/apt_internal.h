@@ -59,8 +59,10 @@ struct StepArgs {
                            // kRecFlag set: the entries of the point are in record (list position);
                            // preceded by the record header (kFixHeader)
     // Counters of this step: fix_count[0] entries in fix_list; wl_count (below)
-    // points into the same 4-word set.  Consecutive steps alternate between two
-    // sets; cnt_next is the other one, cleared by this step's plan kernel.
+    // points into the same 4-word set.  Steps take three sets in turn;
+    // cnt_next is the next step's, cleared by this step's plan kernel (its
+    // last user, two steps back, has finished; this step's fix-up, which may
+    // run beside the next step's plan, keeps its own).
     unsigned* fix_count;
     unsigned* cnt_next;
     int* sink_i;           // 64-lane write sinks for lanes that must not store
@@ -135,7 +137,7 @@ struct BatchSim {
     unsigned ep_ctr;       // epoch counter of the simulation
     int cur;               // source buffer of the next step
     int step;              // index s of the next step (t = (float)s*dt*tau0)
-    int parity;            // counter set of the next step
+    int parity;            // counter set of the next step (0..2, in turn)
     unsigned* cnt;         // the simulation's two 4-word counter sets
     float tau0;
     int plan_base;         // first plan wave of this simulation
@@ -251,28 +253,35 @@ hipError_t launch_plan(const StepArgs& a, hipStream_t stream);
 hipError_t launch_batch_plan(const BatchArgs& b, hipStream_t stream);
 hipError_t launch_batch_advance(const BatchArgs& b, hipStream_t stream);
 hipError_t launch_step_k5_g7(const StepArgs& a, hipStream_t stream, const EdgeLaunch* edge,
-                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3);
+                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3,
+                             bool planned, const StepArgs* next);
 hipError_t launch_batch_k5_g7(const BatchArgs& b, hipStream_t stream);
 int step_slots_k5_g7(int device, int which);
 hipError_t launch_step_k5_g27(const StepArgs& a, hipStream_t stream, const EdgeLaunch* edge,
-                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3);
+                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3,
+                             bool planned, const StepArgs* next);
 hipError_t launch_batch_k5_g27(const BatchArgs& b, hipStream_t stream);
 int step_slots_k5_g27(int device, int which);
 hipError_t launch_step_k8_g7(const StepArgs& a, hipStream_t stream, const EdgeLaunch* edge,
-                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3);
+                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3,
+                             bool planned, const StepArgs* next);
 hipError_t launch_batch_k8_g7(const BatchArgs& b, hipStream_t stream);
 int step_slots_k8_g7(int device, int which);
 hipError_t launch_step_k8_g27(const StepArgs& a, hipStream_t stream, const EdgeLaunch* edge,
-                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3);
+                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3,
+                             bool planned, const StepArgs* next);
 hipError_t launch_batch_k8_g27(const BatchArgs& b, hipStream_t stream);
 int step_slots_k8_g27(int device, int which);
-// Work-list plan, main step kernel, then the overflow fix-up kernels.
+// Work-list plan (unless `planned`: the previous launch's apt_fixplan made
+// it), main step kernel, then the overflow fix-up, or with `next` the fix-up
+// together with the plan of the step `next` describes (apt_fixplan).
 // Optional events are recorded before the plan, after it, after the main
-// kernel and after the fix-ups.
+// kernel and after the fix-up.
 hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,
                        const EdgeLaunch* edge, hipEvent_t ev_begin = nullptr,
                        hipEvent_t ev_planned = nullptr, hipEvent_t ev_mid = nullptr,
-                       hipEvent_t ev_end = nullptr);
+                       hipEvent_t ev_end = nullptr, bool planned = false,
+                       const StepArgs* next = nullptr);
 // Device-side QoIs on a collected alpha field (qoi.hip).
 hipError_t launch_qoi_volume(const int* alpha, int fnx, int fny, int fnz, int max_id, int* counts,
                              int* first, unsigned long long* above, hipStream_t stream);

@@ -12,47 +12,6 @@ namespace {
 // 16 for a batch, whose plan has many more waves.
 constexpr int kBatchPlanBlock = 1024;
 
-// List position of this lane's chunk if `put`: a heavy chunk (more than half
-// of its planes to process) from the front of the list, a light one from the
-// back (cap - 1 downward).  Every thread of the workgroup (kPlanWaves waves)
-// calls it; waves take consecutive runs of positions in wave order.
-template <int kPlanWaves>
-__device__ __forceinline__ unsigned plan_slot(bool put, bool heavy, unsigned* front, unsigned* back,
-                                              unsigned cap, unsigned* s_cnt)
-{
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const unsigned long long mh = __ballot(put && heavy);
-    const unsigned long long ml = __ballot(put && !heavy);
-    if (lane == 0) {
-        s_cnt[w] = (unsigned)__popcll(mh);
-        s_cnt[kPlanWaves + 1 + w] = (unsigned)__popcll(ml);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned th = 0u, tl = 0u;
-        for (int u = 0; u < kPlanWaves; ++u) {
-            const unsigned ch = s_cnt[u], cl = s_cnt[kPlanWaves + 1 + u];
-            s_cnt[u] = th;
-            s_cnt[kPlanWaves + 1 + u] = tl;
-            th += ch;
-            tl += cl;
-        }
-        s_cnt[kPlanWaves] = th ? atomicAdd(front, th) : 0u;
-        s_cnt[2 * kPlanWaves + 1] = tl ? atomicAdd(back, tl) : 0u;
-    }
-    __syncthreads();
-    const unsigned long long below = (1ull << lane) - 1ull;
-    if (heavy) return s_cnt[kPlanWaves] + s_cnt[w] + (unsigned)__popcll(mh & below);
-    return cap - 1u - (s_cnt[2 * kPlanWaves + 1] + s_cnt[kPlanWaves + 1 + w] +
-                       (unsigned)__popcll(ml & below));
-}
-
-__device__ __forceinline__ bool plan_heavy(uint64_t need, int kchunk)
-{
-    return 2 * __popcll(need) > kchunk;
-}
-
 // Work list of one step (runs before apt_step_tiled): waves over (z-chunk
 // group, tiles), see plan_wave.  A tile chunk is listed only if some plane of
 // it needs processing, with its plane masks.  Edge items are not listed: the
@@ -60,26 +19,7 @@ __device__ __forceinline__ bool plan_heavy(uint64_t need, int kchunk)
 // points decide one by one), or the edge kernel's in overflow steps.
 __global__ __launch_bounds__(kBlock) void apt_plan(StepArgs a)
 {
-    constexpr int kPlanWaves = kBlock / 64;
-    __shared__ unsigned s_cnt[2 * kPlanWaves + 2];
-    const int g = blockIdx.x * kBlock + threadIdx.x;
-    // the counter set of the next step (last used by the previous one, whose
-    // fix-up has finished) starts from zero
-    if (g < 4) a.cnt_next[g] = 0u;
-    const int gw = g >> 6;  // wave-uniform
-    const int lane = threadIdx.x & 63;
-    bool put = false;
-    int item = 0;
-    uint64_t need = 0ull, act = 0ull;
-    if (gw < plan_waves(a.kchunk, a.nzchunk, a.items_chunk))
-        plan_wave(a, gw, lane, put, item, need, act);
-    const unsigned q = plan_slot<kPlanWaves>(put, plan_heavy(need, a.kchunk), a.wl_count, a.wl_light,
-                                             (unsigned)(a.nzchunk * a.items_chunk), s_cnt);
-    if (put) {
-        a.wl_item[q] = item;
-        a.wl_need[q] = need;
-        a.wl_act[q] = act;
-    }
+    plan_block(a, (int)blockIdx.x);
 }
 
 // Batched work list: every simulation's plan waves in turn; plan_base counts
@@ -122,8 +62,9 @@ __global__ __launch_bounds__(kBatchPlanBlock) void batch_plan(BatchArgs b)
 
 // After a batched step: every simulation moves to its next step the way the
 // host does between single launches (buffers swap, the destination takes the
-// step's epoch, the step index and t advance, the other counter set is
-// cleared for the next step); the batch's own list counters flip the same way.
+// step's epoch, the step index and t advance, the next of the member's three
+// counter sets is cleared for the next step); the batch's own two list
+// counter sets alternate.
 __global__ __launch_bounds__(64) void batch_advance(BatchArgs b)
 {
     const int s = blockIdx.x * 64 + threadIdx.x;
@@ -139,7 +80,7 @@ __global__ __launch_bounds__(64) void batch_advance(BatchArgs b)
     m.ep[1 - m.cur] = m.a.ep_new;
     m.cur = 1 - m.cur;
     m.step += 1;
-    m.parity = 1 - m.parity;
+    m.parity = (m.parity + 1) % 3;
     unsigned* next = m.cnt + 4 * m.parity;
     next[0] = 0u;
     next[1] = 0u;
@@ -157,7 +98,7 @@ __global__ __launch_bounds__(64) void batch_advance(BatchArgs b)
     m.ep_ctr += 1u;
     a.ep_new = m.ep_ctr;
     a.fix_count = next;
-    a.cnt_next = m.cnt + 4 * (1 - m.parity);
+    a.cnt_next = m.cnt + 4 * ((m.parity + 1) % 3);
     a.wl_count = next + 2;
     a.wl_light = next + 1;
     // t_cur_step = (2*kt+1)*dt*tau0 in float (3DgrainMPI/APTPhaseField.cu:863,919)
@@ -482,14 +423,14 @@ hipError_t launch_batch_advance(const BatchArgs& b, hipStream_t stream)
 
 hipError_t launch_step(const StepArgs& a, int K, int gather27, hipStream_t stream,
                        const EdgeLaunch* edge, hipEvent_t ev_begin, hipEvent_t ev_planned,
-                       hipEvent_t ev_mid, hipEvent_t ev_end)
+                       hipEvent_t ev_mid, hipEvent_t ev_end, bool planned, const StepArgs* next)
 {
     if (K == 5)
-        return gather27 ? launch_step_k5_g27(a, stream, edge, ev_begin, ev_planned, ev_mid, ev_end)
-                        : launch_step_k5_g7(a, stream, edge, ev_begin, ev_planned, ev_mid, ev_end);
+        return gather27 ? launch_step_k5_g27(a, stream, edge, ev_begin, ev_planned, ev_mid, ev_end, planned, next)
+                        : launch_step_k5_g7(a, stream, edge, ev_begin, ev_planned, ev_mid, ev_end, planned, next);
     if (K == 8)
-        return gather27 ? launch_step_k8_g27(a, stream, edge, ev_begin, ev_planned, ev_mid, ev_end)
-                        : launch_step_k8_g7(a, stream, edge, ev_begin, ev_planned, ev_mid, ev_end);
+        return gather27 ? launch_step_k8_g27(a, stream, edge, ev_begin, ev_planned, ev_mid, ev_end, planned, next)
+                        : launch_step_k8_g7(a, stream, edge, ev_begin, ev_planned, ev_mid, ev_end, planned, next);
     return hipErrorInvalidValue;
 }
 

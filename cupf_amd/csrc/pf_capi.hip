@@ -53,7 +53,7 @@ struct pf_ctx {
     unsigned* fix_list = nullptr;   // overflow fix-up list, one entry per interior point
     int* rec = nullptr;             // fix-up records: 7 x K ids, then 7 x K phis (structure of arrays)
     unsigned rec_cap = 0;           // records: min(interior points / 8, 2^20), none for 27-point
-    unsigned* fix_count = nullptr;  // two 4-word counter sets, used by alternate steps
+    unsigned* fix_count = nullptr;  // three 4-word counter sets, used by steps in turn
     int* sink = nullptr;            // 128 words written by lanes that must not store
     long long* sums = nullptr;
     std::vector<long long> h_sums;
@@ -217,9 +217,13 @@ StepArgs make_step_args(pf_ctx* c, int src, float t, int use_iuc)
     a.overflow = c->counters;
     a.fix_list = c->fix_list;
     {
-        const int set = (int)(c->nlaunch & 1ull);  // advanced once the launch succeeded
+        // three sets in turn (advanced once the launch succeeded): this step's
+        // plan clears the next step's set, whose last user (two steps back) has
+        // finished; the next step's plan may run beside this step's fix-up
+        // (apt_fixplan), so it must not clear this step's set
+        const int set = (int)(c->nlaunch % 3ull);
         a.fix_count = c->fix_count + 4 * set;
-        a.cnt_next = c->fix_count + 4 * (1 - set);
+        a.cnt_next = c->fix_count + 4 * ((set + 1) % 3);
         a.wl_count = a.fix_count + 2;
         a.wl_light = a.fix_count + 1;
     }
@@ -479,7 +483,7 @@ int batch_upload(pf_batch* b, const std::vector<int>& first_step, int use_iuc, B
         m.ep_ctr = c->ep_ctr;
         m.cur = c->cur;
         m.step = first_step[i];
-        m.parity = (int)(c->nlaunch & 1ull);
+        m.parity = (int)(c->nlaunch % 3ull);
         m.cnt = c->fix_count;
         m.tau0 = p.tau0;
         m.plan_base = plan_total;
@@ -703,7 +707,7 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
         (rc = dalloc(&c->phi, (size_t)L)) || (rc = dalloc(&c->alpha, (size_t)L)) ||
         (rc = dalloc(&c->counters, 2)) || (rc = dalloc(&c->sums, p.fnz)) ||
         (rc = dalloc(&c->fix_buf, kFixHeader + (size_t)(c->fix_alloc = (long long)(p.fnx - 2) * (p.fny - 2) * (p.fnz - 2)))) ||
-        (rc = dalloc(&c->fix_count, 8)) || (rc = dalloc(&c->sink, 128)) ||
+        (rc = dalloc(&c->fix_count, 12)) || (rc = dalloc(&c->sink, 128)) ||
         (rc = dalloc(&c->act[0], (size_t)c->ntiles * p.fnz)) ||
         (rc = dalloc(&c->act[1], (size_t)c->ntiles * p.fnz)) ||
         (rc = dalloc(&c->wl_item, (size_t)(c->wl_alloc = (long long)c->nzchunk * c->items_chunk + 1))) ||
@@ -728,7 +732,7 @@ int pf_create(const pf_params* params, int device, pf_ctx** out)
             return bail(fail(PF_E_HIP, "initial fill failed"));
     }
     if (hipMemsetAsync(c->counters, 0, 2 * sizeof(unsigned long long), c->stream) != hipSuccess ||
-        hipMemsetAsync(c->fix_count, 0, 8 * sizeof(unsigned), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->fix_count, 0, 12 * sizeof(unsigned), c->stream) != hipSuccess ||
         hipMemcpyAsync(c->fix_list - 4, hdr, sizeof(hdr), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         hipMemsetAsync(c->alpha, 0, L * sizeof(int), c->stream) != hipSuccess ||
         hipMemsetAsync(c->act[0], 0, (size_t)c->ntiles * p.fnz * sizeof(unsigned), c->stream) != hipSuccess ||
@@ -935,16 +939,37 @@ static int step_impl(pf_ctx* c, int nsteps, int first_step, int flags)
     if (nsteps < 0) return fail(PF_E_INVALID, "nsteps must be >= 0");
     const int use_iuc = (flags & PF_STEP_INITIAL_UC) ? 1 : 0;
     const bool nucl = (flags & PF_STEP_NUCLEATE) != 0;
-    for (int s = first_step; s < first_step + nsteps; ++s) {
+    const int end = first_step + nsteps;
+    // Without nucleation (which runs between steps), each step's fix-up and
+    // the next step's plan go out as one launch (apt_fixplan): the next
+    // step's arguments are made before this step is launched, from the host
+    // state as it stands after this launch.  The last step of the call ends
+    // with its own fix-up, so no work is left pending between calls.
+    StepArgs a{};
+    bool have = false, planned = false;
+    for (int s = first_step; s < end; ++s) {
         // t_cur_step = (2*kt+1)*dt*tau0 in float (3DgrainMPI/APTPhaseField.cu:863,919).
         const float t = s * p.dt * p.tau0;
-        HIPCHK(act_renew_if_needed(c));
-        HIPCHK(bound_lead(c));
-        StepArgs a = make_step_args(c, c->cur, t, use_iuc);
-        HIPCHK(launch_step(a, p.K, p.gather27, c->stream, &c->edge));
+        if (!have) {
+            HIPCHK(act_renew_if_needed(c));
+            HIPCHK(bound_lead(c));
+            a = make_step_args(c, c->cur, t, use_iuc);
+            planned = false;
+        }
+        const StepArgs cur = a;
         ++c->nlaunch;
-        c->ep[1 - c->cur] = a.ep_new;
+        c->ep[1 - c->cur] = cur.ep_new;
         c->cur = 1 - c->cur;
+        // an activity-map renewal (epoch counter near its end) must come
+        // between this step and the next plan: no fused launch then
+        have = !nucl && s + 1 < end && c->ep_ctr < 0x7fffffffu - 1u;
+        if (have) {
+            HIPCHK(bound_lead(c));
+            a = make_step_args(c, c->cur, (s + 1) * p.dt * p.tau0, use_iuc);
+        }
+        HIPCHK(launch_step(cur, p.K, p.gather27, c->stream, &c->edge, nullptr, nullptr, nullptr,
+                           nullptr, planned, have ? &a : nullptr));
+        planned = have;
         if (nucl && (s & 1)) {
             int rc = nucl_launch(c, t, 2.0f * p.dt * p.tau0, use_iuc);
             if (rc) return rc;

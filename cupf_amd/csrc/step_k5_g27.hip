@@ -4,9 +4,10 @@
 namespace cupf {
 
 hipError_t launch_step_k5_g27(const StepArgs& a, hipStream_t stream, const EdgeLaunch* edge,
-                              hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3)
+                             hipEvent_t e0, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3,
+                             bool planned, const StepArgs* next)
 {
-    return launch_step_t<5, true>(a, stream, edge, e0, e1, e2, e3);
+    return launch_step_t<5, true>(a, stream, edge, e0, e1, e2, e3, planned, next);
 }
 
 hipError_t launch_batch_k5_g27(const BatchArgs& b, hipStream_t stream)

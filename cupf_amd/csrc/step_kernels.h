@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void apt_edge(StepArgs a)
 // lives in this step's counter set, which the next-but-one plan kernel
 // clears, so nothing here has to wait for the other workgroups.
 template <int K, bool G27>
-__global__ __launch_bounds__(kFixLanes) void apt_fixup(StepArgs a_arg)
+__device__ __forceinline__ void fixup_run(const StepArgs& a_arg, int blk, int nblk)
 {
     // The arguments are read from an LDS copy: with them in scalar registers
     // the compiler keeps 229 VGPRs live (2 waves/SIMD); from LDS it needs 92
@@ -115,22 +115,43 @@ __global__ __launch_bounds__(kFixLanes) void apt_fixup(StepArgs a_arg)
     __syncthreads();
     const StepArgs& a = s_a;
     const unsigned n = __atomic_load_n(a.fix_count, __ATOMIC_RELAXED);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && a.seen) {  // for the host's variant choice
+    if (blk == 0 && threadIdx.x == 0 && a.seen) {  // for the host's variant choice
         if (n >= kOvfHeavy) a.seen[1] = a.launch_index;
         a.seen[0] = a.launch_index;
     }
     if (n == 0u) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.overflow, (unsigned long long)n);
+    if (blk == 0 && threadIdx.x == 0) atomicAdd(a.overflow, (unsigned long long)n);
     const float a_s = 1.0f + 3.0f * a.kin_delta;
     const float epsv = -4.0f * a.kin_delta / a_s;
 #pragma unroll 1
-    for (unsigned q = blockIdx.x * kFixLanes + threadIdx.x; q < n; q += gridDim.x * kFixLanes) {
+    for (unsigned q = blk * kFixLanes + threadIdx.x; q < n; q += nblk * kFixLanes) {
         int i, j, k;
         bool rec;
         fixup_point(a, q, i, j, k, rec);
         if (!G27 && rec) apt_point_overflow_rec<K>(a, i, j, k, (int)q, a_s, epsv);
         else apt_point_overflow<K, G27>(a, i, j, k, a_s, epsv);
     }
+}
+
+template <int K, bool G27>
+__global__ __launch_bounds__(kFixLanes) void apt_fixup(StepArgs a)
+{
+    fixup_run<K, G27>(a, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// The fix-up of one step and the work-list plan of the next in one launch:
+// blocks [0, fa.fix_blocks) run the fix-up with fa, the others the plan with
+// pa.  The plan reads only activity words, which the fix-up does not write
+// (the step kernel marks an overflowed point's tile), and the fix-up reads
+// its own step's counter set while the plan fills the next one and clears
+// the one after (three sets in turn), so the two halves need no order
+// between them.  Saves one launch and its dispatch gap per step.
+template <int K, bool G27>
+__global__ __launch_bounds__(kBlock) void apt_fixplan(StepArgs fa, StepArgs pa)
+{
+    static_assert(kFixLanes == kBlock, "one workgroup size for both halves");
+    if ((int)blockIdx.x < fa.fix_blocks) fixup_run<K, G27>(fa, (int)blockIdx.x, fa.fix_blocks);
+    else plan_block(pa, (int)blockIdx.x - fa.fix_blocks);
 }
 
 // Index of the simulation whose range [base_i, base_{i+1}) holds work index q
@@ -254,15 +275,15 @@ __global__ __launch_bounds__(kFixLanes) void batch_fixup(BatchArgs b)
 template <int K, bool G27>
 hipError_t launch_step_t(const StepArgs& a, hipStream_t stream, const EdgeLaunch* edge,
                          hipEvent_t ev_begin, hipEvent_t ev_planned, hipEvent_t ev_mid,
-                         hipEvent_t ev_end)
+                         hipEvent_t ev_end, bool planned, const StepArgs* next)
 {
     dim3 grid(a.step_blocks), block(kBlock), fgrid(a.fix_blocks), fblock(kFixLanes);
     // overflow steps (7-point gather) run their edge items as apt_edge
     const int ne = (!G27 && a.ovf_variant) ? a.nzchunk * a.edge_items : 0;
     if (ne > 0 && !edge) return hipErrorInvalidValue;
     if (ev_begin) (void)hipEventRecord(ev_begin, stream);
-    hipError_t e = launch_plan(a, stream);
-    if (e != hipSuccess) return e;
+    hipError_t e;
+    if (!planned && (e = launch_plan(a, stream)) != hipSuccess) return e;
     if (ev_planned) (void)hipEventRecord(ev_planned, stream);
     if (ne > 0) {
         if ((e = hipEventRecord(edge->ready, stream)) != hipSuccess ||
@@ -277,7 +298,12 @@ hipError_t launch_step_t(const StepArgs& a, hipStream_t stream, const EdgeLaunch
         hipLaunchKernelGGL((apt_step_tiled<K, G27, false>), grid, block, 0, stream, a);
     if (ne > 0 && (e = hipStreamWaitEvent(stream, edge->done, 0)) != hipSuccess) return e;
     if (ev_mid) (void)hipEventRecord(ev_mid, stream);
-    hipLaunchKernelGGL((apt_fixup<K, G27>), fgrid, fblock, 0, stream, a);
+    if (next) {  // this step's fix-up and the next step's plan in one launch
+        const int pblocks = max(1, cdiv(plan_waves(next->kchunk, next->nzchunk, next->items_chunk), kBlock / 64));
+        hipLaunchKernelGGL((apt_fixplan<K, G27>), dim3(a.fix_blocks + pblocks), block, 0, stream, a, *next);
+    } else {
+        hipLaunchKernelGGL((apt_fixup<K, G27>), fgrid, fblock, 0, stream, a);
+    }
     if (ev_end) (void)hipEventRecord(ev_end, stream);
     return hipGetLastError();
 }
