@@ -129,9 +129,11 @@ int pf_step(pf_ctx* ctx, int nsteps, int first_step, int flags);
 int pf_step_timed(pf_ctx* ctx, int nsteps, int first_step, int flags, float* ms);
 
 /* Diagnostic: pf_step with HIP events on the context's stream around every
- * launch of the work-list plan kernel, of the main step kernel and of the
- * overflow fix-up kernels; the summed elapsed times are returned in ms (any
- * pointer may be NULL).  No nucleation.  No counterpart in the reference (its
+ * launch of the work-list plan kernel, of the main step kernel (with, in
+ * steps whose neighbourhoods overflow, the edge kernel that runs beside it on
+ * a second stream) and of the overflow fix-up kernel, each launched on its own
+ * (pf_step sends a fix-up and the next plan as one launch); the summed
+ * elapsed times are returned in ms (any pointer may be NULL).  No nucleation.  No counterpart in the reference (its
  * only timer is the wall clock around the loop, 3DgrainMPI/APTPhaseField.cu:850,940). */
 int pf_step_profiled(pf_ctx* ctx, int nsteps, int first_step, int flags, float* ms_main,
                      float* ms_fixup, float* ms_plan);
