@@ -155,3 +155,23 @@ def test_batch_variants_same_bits(K, monkeypatch):
             assert ctx.overflow_count() == alone[q][1]
             ctx.close()
     monkeypatch.delenv("CUPF_STEP_VARIANT", raising=False)
+
+
+def test_fused_fixup_plan_same_bits():
+    """pf_step sends each step's fix-up and the next step's plan as one launch
+    within a call; one step per call, and pf_step_profiled, launch them
+    separately.  All three end with the same bits and overflow counts,
+    through the overflow start-up (one-column +x strip, edge kernel)."""
+    c = _dense_case("fused", (64, 40, 24), seed=41)
+    ctxs = [gpu_context(c) for _ in range(3)]
+    n = 60
+    ctxs[0].step(n, 1)
+    for s in range(1, n + 1):
+        ctxs[1].step(1, s)
+    ctxs[2].step_profiled(n, 1)
+    got = [(ensemble.state_hash(x), x.overflow_count()) for x in ctxs]
+    print(f"[fused] overflow {[g[1] for g in got]}", flush=True)
+    assert got[0][1] > 0
+    assert got[1] == got[0] and got[2] == got[0]
+    for x in ctxs:
+        x.close()
